@@ -1,0 +1,273 @@
+"""doci_amd -- DOCI ground-state solver on one AMD Instinct MI355X.
+
+Python front-end of libdoci_amd.so (C host layer + HIP kernels for gfx950).
+The library is loaded from this package directory; there is no fallback:
+importing works without a GPU, but every compute call goes through HIP and
+raises DociError when the library or the device is missing.
+
+The C entry points mirror klgunst/doci's (fill_doci_data, doci_matvec,
+sparse_eigensolve, ...); see include/doci_amd.h.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+
+import numpy as np
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libdoci_amd.so")
+CLI_PATH = os.path.join(PKG, "bin", "doci")
+
+_lib = None
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+
+
+class DociError(RuntimeError):
+    pass
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [("solver", C.c_char * 8), ("tol", C.c_double), ("max_its", C.c_int),
+                ("davidson_keep", C.c_int), ("davidson_max_vec", C.c_int),
+                ("precision", C.c_int), ("switch_tol", C.c_double), ("trace", C.c_int)]
+
+
+class SolveInfo(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("iterations_fp32", C.c_int),
+                ("sigma_calls", C.c_int), ("converged", C.c_int), ("residual", C.c_double),
+                ("energy", C.c_double), ("seconds", C.c_double),
+                ("seconds_sigma", C.c_double), ("sigma_calls_fp32", C.c_int),
+                ("seconds_sigma_fp32", C.c_double)]
+
+
+class InputFile(C.Structure):
+    _fields_ = [("fcidump", C.c_char * 255), ("solver", C.c_char * 20), ("max_its", C.c_int),
+                ("tol", C.c_double), ("davidson_keep", C.c_int),
+                ("davidson_max_vec", C.c_int), ("hf_init", C.c_int), ("precision", C.c_int)]
+
+
+# Every symbol include/doci_amd.h declares (checked by the CPU test suite).
+EXPORTS = [
+    "read_fcidump", "fill_doci_data", "doci_get_diagonal", "doci_get_basis_size",
+    "doci_cleanup_data", "doci_matvec", "sparse_eigensolve", "davidson", "ray_q",
+    "doci_last_error", "doci_op_from_fcidump", "doci_op_from_integrals", "doci_op_free",
+    "doci_op_dim", "doci_op_L", "doci_op_N", "doci_op_diagonal", "doci_op_sigma",
+    "doci_op_sigma_dev", "doci_op_stream", "doci_solve_defaults", "doci_solve",
+    "doci_write_fcidump", "doci_gen_integrals", "doci_gen_vector", "doci_read_inputfile",
+    "doci_read_fcidump", "doci_free",
+]
+
+
+def lib():
+    """The loaded libdoci_amd.so (raises DociError when it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise DociError(f"{LIB_PATH} is not built: run `python -m doci_amd.build`")
+    l = C.CDLL(LIB_PATH)
+    vp = C.c_void_p
+    l.doci_last_error.restype = C.c_char_p
+    l.doci_op_from_fcidump.argtypes = [C.c_char_p, C.POINTER(vp), C.POINTER(C.c_int)]
+    l.doci_op_from_integrals.argtypes = [C.c_uint, C.c_uint, C.c_double, _dp, _dp, C.POINTER(vp)]
+    l.doci_op_free.argtypes = [vp]
+    l.doci_op_dim.argtypes = [vp]
+    l.doci_op_dim.restype = C.c_int64
+    l.doci_op_L.argtypes = [vp]
+    l.doci_op_L.restype = C.c_uint
+    l.doci_op_N.argtypes = [vp]
+    l.doci_op_N.restype = C.c_uint
+    l.doci_op_diagonal.argtypes = [vp, _dp]
+    l.doci_op_sigma.argtypes = [vp, _dp, _dp]
+    l.doci_op_sigma_dev.argtypes = [vp, vp, vp, C.c_int, C.c_double]
+    l.doci_op_stream.argtypes = [vp]
+    l.doci_op_stream.restype = vp
+    l.doci_solve_defaults.argtypes = [C.POINTER(SolveOpts)]
+    l.doci_solve.argtypes = [vp, C.POINTER(SolveOpts), vp, vp, C.POINTER(C.c_double),
+                             C.POINTER(SolveInfo)]
+    l.doci_write_fcidump.argtypes = [C.c_char_p, C.c_uint, C.c_uint, C.c_double, _dp, _dp]
+    l.doci_gen_integrals.argtypes = [C.c_uint, C.c_uint64, C.POINTER(C.c_double), _dp, _dp]
+    l.doci_gen_vector.argtypes = [C.c_int64, C.c_uint64, _dp]
+    l.doci_read_inputfile.argtypes = [C.c_char_p, C.POINTER(InputFile)]
+    l.doci_read_fcidump.argtypes = [C.c_char_p, C.POINTER(C.c_uint), C.POINTER(C.c_uint),
+                                    C.POINTER(C.c_uint), C.POINTER(C.c_double),
+                                    C.POINTER(C.POINTER(C.c_double)),
+                                    C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.c_int)]
+    l.doci_free.argtypes = [vp]
+    _lib = l
+    return l
+
+
+def _check(rc: int, what: str) -> int:
+    if rc < 0:
+        msg = lib().doci_last_error().decode(errors="replace")
+        raise DociError(f"{what} failed ({rc}): {msg}")
+    return rc
+
+
+# ---------------------------------------------------------------------------
+# host-side helpers (no GPU needed)
+# ---------------------------------------------------------------------------
+
+def gen_integrals(L: int, seed: int):
+    """Seeded synthetic integrals (core, one[L*L], two[L^4]) in the raw
+    FCIDUMP layout (canonical entries only)."""
+    one = np.zeros(L * L)
+    two = np.zeros(L ** 4)
+    core = C.c_double()
+    _check(lib().doci_gen_integrals(L, seed, C.byref(core), one, two), "doci_gen_integrals")
+    return core.value, one, two
+
+
+def write_fcidump(path: str, L: int, N: int, core: float, one, two) -> None:
+    _check(lib().doci_write_fcidump(path.encode(), L, N, core, np.ascontiguousarray(one),
+                                    np.ascontiguousarray(two)), "doci_write_fcidump")
+
+
+def gen_fcidump(path: str, L: int, N: int, seed: int) -> None:
+    core, one, two = gen_integrals(L, seed)
+    write_fcidump(path, L, N, core, one, two)
+
+
+def read_fcidump(path: str):
+    """(L, Na, Nb, core, one[L*L], two[L^4]) as read (untransformed); raises
+    DociError with the offending line number for malformed input."""
+    L, Na, Nb = C.c_uint(), C.c_uint(), C.c_uint()
+    core = C.c_double()
+    one, two = C.POINTER(C.c_double)(), C.POINTER(C.c_double)()
+    line = C.c_int(0)
+    rc = lib().doci_read_fcidump(path.encode(), C.byref(L), C.byref(Na), C.byref(Nb),
+                                 C.byref(core), C.byref(one), C.byref(two), C.byref(line))
+    if rc < 0:
+        err = DociError(f"doci_read_fcidump({path}) failed ({rc}): "
+                        + lib().doci_last_error().decode(errors="replace"))
+        err.code, err.line = rc, line.value
+        raise err
+    l = L.value
+    o = np.ctypeslib.as_array(one, (l * l,)).copy()
+    t = np.ctypeslib.as_array(two, (l ** 4,)).copy()
+    lib().doci_free(C.cast(one, C.c_void_p))
+    lib().doci_free(C.cast(two, C.c_void_p))
+    return l, Na.value, Nb.value, core.value, o, t
+
+
+def gen_vector(n: int, seed: int) -> np.ndarray:
+    v = np.zeros(n)
+    _check(lib().doci_gen_vector(n, seed, v), "doci_gen_vector")
+    return v
+
+
+def read_inputfile(path: str) -> dict:
+    s = InputFile()
+    _check(lib().doci_read_inputfile(path.encode(), C.byref(s)), "doci_read_inputfile")
+    return dict(fcidump=s.fcidump.decode(), solver=s.solver.decode(), max_its=s.max_its,
+                tol=s.tol, davidson_keep=s.davidson_keep,
+                davidson_max_vec=s.davidson_max_vec, hf_init=s.hf_init, precision=s.precision)
+
+
+def binom(k: int, n: int) -> int:
+    from math import comb
+    return comb(n, k)
+
+
+# ---------------------------------------------------------------------------
+# device-resident operator
+# ---------------------------------------------------------------------------
+
+@dataclass
+class Result:
+    energy: float
+    code: int
+    x: np.ndarray | None
+    info: SolveInfo
+
+
+class Operator:
+    """A DOCI Hamiltonian resident on the GPU (tables, diagonal)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        self.n = int(lib().doci_op_dim(handle))
+        self.L = int(lib().doci_op_L(handle))
+        self.N = int(lib().doci_op_N(handle))
+
+    @classmethod
+    def from_fcidump(cls, path: str) -> "Operator":
+        h = C.c_void_p()
+        line = C.c_int(0)
+        _check(lib().doci_op_from_fcidump(path.encode(), C.byref(h), C.byref(line)),
+               f"doci_op_from_fcidump({path})")
+        return cls(h)
+
+    @classmethod
+    def from_integrals(cls, L: int, N: int, core: float, one, two) -> "Operator":
+        h = C.c_void_p()
+        _check(lib().doci_op_from_integrals(L, N, core, np.ascontiguousarray(one, dtype=np.float64),
+                                            np.ascontiguousarray(two, dtype=np.float64),
+                                            C.byref(h)), "doci_op_from_integrals")
+        return cls(h)
+
+    @classmethod
+    def synthetic(cls, L: int, N: int, seed: int) -> "Operator":
+        core, one, two = gen_integrals(L, seed)
+        return cls.from_integrals(L, N, core, one, two)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def diagonal(self) -> np.ndarray:
+        out = np.zeros(self.n)
+        _check(lib().doci_op_diagonal(self._h, out), "doci_op_diagonal")
+        return out
+
+    def sigma(self, c: np.ndarray) -> np.ndarray:
+        out = np.zeros(self.n)
+        _check(lib().doci_op_sigma(self._h, np.ascontiguousarray(c, dtype=np.float64), out),
+               "doci_op_sigma")
+        return out
+
+    def sigma_dev(self, d_in: int, d_out: int, prec: int = 0, shift: float = 0.0) -> None:
+        """Device pointers (ints), on the operator's stream."""
+        _check(lib().doci_op_sigma_dev(self._h, d_in, d_out, prec, shift), "doci_op_sigma_dev")
+
+    def stream(self) -> int:
+        return int(lib().doci_op_stream(self._h) or 0)
+
+    def solve(self, solver: str = "D", tol: float = 1e-8, max_its: int = 200, keep: int = 2,
+              max_vec: int = 30, precision: str = "fp64", switch_tol: float = 1e-5,
+              guess: np.ndarray | None = None, want_x: bool = True, trace: bool = False) -> Result:
+        o = SolveOpts()
+        lib().doci_solve_defaults(C.byref(o))
+        o.solver = solver.encode()
+        o.tol, o.max_its, o.davidson_keep, o.davidson_max_vec = tol, max_its, keep, max_vec
+        o.precision = 1 if precision == "mixed" else 0
+        o.switch_tol = switch_tol
+        o.trace = int(trace)
+        info = SolveInfo()
+        e = C.c_double()
+        x = np.zeros(self.n) if want_x else None
+        g = None if guess is None else np.ascontiguousarray(guess, dtype=np.float64)
+        rc = lib().doci_solve(self._h, C.byref(o), None if g is None else g.ctypes.data,
+                              None if x is None else x.ctypes.data, C.byref(e), C.byref(info))
+        _check(rc, "doci_solve")
+        return Result(e.value, rc, x, info)
+
+    def close(self):
+        if self._h:
+            lib().doci_op_free(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,101 @@
+/*
+ * dk.h -- the thin C ABI between the C host code and the HIP kernels.
+ *
+ * Every function takes device pointers, sizes and a stream, launches one or
+ * more kernels on that stream and returns a hipError_t value (0 = success).
+ * No function here synchronises unless its comment says so.
+ */
+#ifndef DOCI_DK_H
+#define DOCI_DK_H
+
+#include <stddef.h>
+#include <stdint.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DK_MAX_L 64
+#define DK_MAX_VEC 64 /* most vectors one fused Davidson kernel addresses */
+
+/* Device-resident integral and addressing tables of one DOCI operator. */
+typedef struct {
+    uint32_t L, N;
+    uint32_t n;              /* C(L,N) < 2^31 */
+    double core;
+    const uint32_t *binom;   /* binom[x*(N+2)+j] = C(x,j) mod 2^32, x <= L */
+    const double *K;         /* K[o*L+t] = (ot|ot), zero diagonal (off-diagonal H) */
+    const double *J;         /* J[p*L+q] = (pp|qq) */
+    const double *X;         /* X[p*L+q] = (pq|qp) */
+    const double *h;         /* h[p] = h'_pp */
+} dk_tables;
+
+/* Diagonal, bit-identical to the reference's addition order. */
+int dk_diagonal(const dk_tables *t, double *D, hipStream_t s);
+
+/* sigma = (H - shift) c.  prec: 0 = FP64 vectors, 1 = FP32 vectors (FP64
+ * sums).  shift = 0 gives H c exactly. */
+int dk_sigma(const dk_tables *t, const double *D, double shift, const void *c,
+             void *sigma, int prec, hipStream_t s);
+
+/* ---- streaming vector kernels (deterministic two-level reductions) -------
+ * Vectors are arrays of n elements; `vecs` lists device pointers.  prec as
+ * above for the stored vectors; reductions and coefficients are FP64.
+ * `red` is a device scratch of dk_red_scratch_doubles(nvals) doubles;
+ * results land in out_dev (device) in the order documented per function. */
+size_t dk_red_scratch_doubles(int nvals);
+
+/* out[0] = <y, y>, out[1+j] = <X_j, y> for j < m. */
+int dk_multidot(int m, void *const *X, const void *y, int64_t n, int prec,
+                double *red, double *out_dev, hipStream_t s);
+
+/* y <- y - sum_j g_j X_j, then out[0] = <y_new, y_new>, out[1+j] = <X_j, y_new>. */
+int dk_ortho(int m, void *const *X, const double *g, void *y, int64_t n,
+             int prec, double *red, double *out_dev, hipStream_t s);
+
+/* Davidson residual step (src/davidson.c:195-227 fused).  W holds
+ * (H - shift) V, so with the Ritz value theta:
+ *   x = sum_j a_j V_j, r = sum_j a_j W_j - (theta - shift) x,
+ *   t = r / max-floor(|D - theta|, 1e-12)  written to `t`,
+ *   out[0] = <r,r>, out[1] = <t,t>, out[2+j] = <V_j, t> (j < m).
+ * When x_out is not NULL the Ritz vector x is also written there. */
+int dk_residual(int m, void *const *V, void *const *W, const double *a,
+                double theta, double shift, const double *D, void *t, void *x_out,
+                int64_t n, int prec, double *red, double *out_dev, hipStream_t s);
+
+/* out = sum_j g_j X_j (out may not alias any X_j); out has precision
+ * prec_out. */
+int dk_lincomb(int m, void *const *X, const double *g, void *out, int64_t n,
+               int prec, int prec_out, hipStream_t s);
+
+/* Conversions and simple maps. */
+int dk_convert(const void *in, int prec_in, void *out, int prec_out, int64_t n,
+               double scale, hipStream_t s);
+
+/* CG (src/CG.c:74-126 fused):
+ * dk_cg_grad: g = 2(Ax - rho x); out[0] = <g,g>; if pinv: g *= pinv;
+ *             out[1] = <g,g>.
+ * dk_cg_dir:  p = q p - g (first: p = -g when first != 0).
+ * dk_cg_gram: out = [x.Ax, p.Ax, x.Ap, p.Ap, x.x, p.x, p.p].
+ * dk_cg_step: x = z0 x + z1 p; Ax = z0 Ax + z1 Ap; then dk_cg_grad on the
+ *             new x, Ax (same outputs). */
+int dk_cg_grad(const void *x, const void *Ax, double rho, const double *pinv,
+               void *g, int64_t n, int prec, double *red, double *out_dev,
+               hipStream_t s);
+int dk_cg_dir(void *p, const void *g, double q, int first, int64_t n, int prec,
+              hipStream_t s);
+int dk_cg_gram(const void *x, const void *p, const void *Ax, const void *Ap,
+               int64_t n, int prec, double *red, double *out_dev, hipStream_t s);
+int dk_cg_step(void *x, const void *p, void *Ax, const void *Ap, double z0,
+               double z1, double rho, const double *pinv, void *g, int64_t n,
+               int prec, double *red, double *out_dev, hipStream_t s);
+/* pinv = 1/D with the reference's 1e-12 floor (CG.c:57-63). */
+int dk_precond_inv(const double *D, double *pinv, int64_t n, hipStream_t s);
+/* index of the first minimum of D (executable.c:276-288); result in *idx_dev. */
+int dk_argmin(const double *D, int64_t n, double *red, int64_t *idx_dev, hipStream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,320 @@
+/* davidson.c -- device-resident Davidson (src/davidson.c:33-255).
+ *
+ * Semantics kept from the reference: the loop test `res > tol && its++ <
+ * max_its`, the |D - theta| preconditioner with its 1e-12 floor, deflation
+ * right after the subspace reaches max_vec keeping the `keep` lowest Ritz
+ * vectors, and the returned code its >= max_its.
+ *
+ * MI355X form: the subspace V and its images W = (H - shift) V live in HBM;
+ * each iteration is four streaming passes (orthogonalise, sigma, one new
+ * column of V^T W, fused Ritz/residual/preconditioner) and only O(m)
+ * scalars cross to the host.  Vectors are stored unnormalised with a scale
+ * per vector (V_j = s_j Vt_j), which removes the normalisation pass.  The
+ * orthogonalisation is classical Gram-Schmidt repeated while the measured
+ * loss of orthogonality exceeds a threshold (the reference uses modified
+ * Gram-Schmidt; both give the same subspace to rounding).  The Rayleigh-Ritz
+ * step solves S y = theta G y with the Gram matrix G = V^T V measured in FP64,
+ * which equals I for FP64 storage and absorbs the rounding of FP32 storage.
+ *
+ * Mixed precision (precision = 1): the subspace is stored in FP32 with FP64
+ * sums and the operator applied as H - shift with shift = min(D) (so stored
+ * W entries are small and FP32 rounding of W stays below the switch
+ * tolerance); once the residual drops below switch_tol the solve restarts in
+ * FP64 from the current Ritz vector and finishes with FP64 residuals and
+ * Rayleigh quotients.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "internal.h"
+
+typedef struct {
+    int prec;
+    size_t es;
+    int nslot;
+    void **Vs, **Ws; /* slot storage */
+    void *block;
+    int *basis;      /* slot index of V_j */
+    int *freelist;
+    int nfree;
+} dav_ws;
+
+static void ws_free(dav_ws *w)
+{
+    if (w->block) hipFree(w->block);
+    free(w->Vs);
+    free(w->Ws);
+    free(w->basis);
+    free(w->freelist);
+    memset(w, 0, sizeof *w);
+}
+
+static int ws_alloc(dav_ws *w, int64_t n, int prec, int nslot)
+{
+    memset(w, 0, sizeof *w);
+    w->prec = prec;
+    w->es = prec ? sizeof(float) : sizeof(double);
+    w->nslot = nslot;
+    const size_t vb = ((size_t)n * w->es + 255) & ~(size_t)255;
+    w->Vs = calloc((size_t)nslot, sizeof(void *));
+    w->Ws = calloc((size_t)nslot, sizeof(void *));
+    w->basis = calloc((size_t)nslot, sizeof(int));
+    w->freelist = calloc((size_t)nslot, sizeof(int));
+    if (!w->Vs || !w->Ws || !w->basis || !w->freelist) {
+        ws_free(w);
+        return doci_fail(DOCI_E_NOMEM, "Davidson host workspace");
+    }
+    hipError_t e = hipMalloc(&w->block, vb * 2 * (size_t)nslot);
+    if (e != hipSuccess) {
+        w->block = NULL;
+        ws_free(w);
+        return doci_fail(DOCI_E_NOMEM, "Davidson subspace of %d x 2 vectors of %lld x %zu B: %s",
+                         nslot, (long long)n, w->es, hipGetErrorString(e));
+    }
+    for (int k = 0; k < nslot; ++k) {
+        w->Vs[k] = (char *)w->block + vb * (size_t)k;
+        w->Ws[k] = (char *)w->block + vb * (size_t)(nslot + k);
+        w->freelist[k] = nslot - 1 - k;
+    }
+    w->nfree = nslot;
+    return DOCI_OK;
+}
+
+static int take_slot(dav_ws *w) { return w->freelist[--w->nfree]; }
+static void give_slot(dav_ws *w, int k) { w->freelist[w->nfree++] = k; }
+
+typedef struct {
+    dh_ctx *c;
+    const doci_solve_opts *o;
+    doci_solve_info *info;
+    int *its;
+    hipEvent_t ev0, ev1;
+} dav_run;
+
+static int timed_apply(dav_run *r, const void *in, void *out, int prec, double shift)
+{
+    HIPCHK(hipEventRecord(r->ev0, r->c->stream));
+    int rc = r->c->apply(r->c, in, out, prec, shift);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(r->ev1, r->c->stream));
+    r->info->sigma_calls++;
+    if (prec) r->info->sigma_calls_fp32++;
+    return DOCI_OK;
+}
+
+static int add_sigma_time(dav_run *r, int prec)
+{
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, r->ev0, r->ev1));
+    r->info->seconds_sigma += 1e-3 * ms;
+    if (prec) r->info->seconds_sigma_fp32 += 1e-3 * ms;
+    return DOCI_OK;
+}
+
+/* One Davidson phase at storage precision `prec`; stops at residual <=
+ * stop_tol.  d_x: FP64 guess in, FP64 Ritz vector out. */
+static int dav_phase(dav_run *r, int prec, double shift, double stop_tol, double *d_x,
+                     double *energy, double *res_out)
+{
+    dh_ctx *c = r->c;
+    const doci_solve_opts *o = r->o;
+    const int mv = o->davidson_max_vec, keep = o->davidson_keep, ld = mv;
+    const int64_t n = c->n;
+    const double ortho_tol = prec ? 1e-6 : 1e-12;
+    dav_ws w;
+    int rc = ws_alloc(&w, n, prec, mv + keep);
+    if (rc) return rc;
+    double *S = calloc((size_t)mv * mv, sizeof(double)), *G = calloc((size_t)mv * mv, sizeof(double));
+    double *Y = calloc((size_t)mv * mv, sizeof(double)), *th = calloc((size_t)mv, sizeof(double));
+    double *s = calloc((size_t)mv + 1, sizeof(double)), *a = calloc((size_t)mv + 2, sizeof(double));
+    double *red = calloc((size_t)mv + 8, sizeof(double)), *g = calloc((size_t)mv + 2, sizeof(double));
+    void **Vp = calloc((size_t)mv + 1, sizeof(void *)), **Wp = calloc((size_t)mv + 1, sizeof(void *));
+    if (!S || !G || !Y || !th || !s || !a || !red || !g || !Vp || !Wp) {
+        rc = doci_fail(DOCI_E_NOMEM, "Davidson host arrays");
+        goto done;
+    }
+#define CK(expr)                       \
+    do {                               \
+        rc = (expr);                   \
+        if (rc) goto done;             \
+    } while (0)
+#define DK(expr)                                                                            \
+    do {                                                                                    \
+        int e_ = (expr);                                                                    \
+        if (e_) {                                                                           \
+            rc = doci_fail(DOCI_E_HIP, "%s: %s", #expr, hipGetErrorString((hipError_t)e_)); \
+            goto done;                                                                      \
+        }                                                                                   \
+    } while (0)
+
+    int m = 0;
+    int tslot = take_slot(&w);
+    double tt;
+    /* initial vector: t = guess, its norm (davidson.c:68 then :144-146) */
+    DK(dk_convert(d_x, 0, w.Vs[tslot], prec, n, 1.0, c->stream));
+    DK(dk_multidot(0, NULL, w.Vs[tslot], n, prec, c->d_red, c->d_out, c->stream));
+    CK(dh_fetch(c, 1, red));
+    tt = red[0];
+    double res = o->tol * 10, theta = 0;
+    const int start_its = *r->its;
+    while (res > stop_tol && (*r->its)++ < o->max_its) {
+        /* orthogonalise t against V (davidson.c:133-176) */
+        for (int j = 0; j < m; ++j) Vp[j] = w.Vs[w.basis[j]];
+        for (int pass = 0; m > 0 && pass < 3; ++pass) {
+            for (int j = 0; j < m; ++j) g[j] = s[j] * s[j] * a[j];
+            DK(dk_ortho(m, Vp, g, w.Vs[tslot], n, prec, c->d_red, c->d_out, c->stream));
+            CK(dh_fetch(c, m + 1, red));
+            tt = red[0];
+            double loss = 0;
+            for (int j = 0; j < m; ++j) {
+                a[j] = red[1 + j];
+                const double cj = fabs(s[j] * a[j]);
+                if (cj > loss) loss = cj;
+            }
+            if (!(tt > 0)) break;
+            if (loss <= ortho_tol * sqrt(tt)) break;
+        }
+        if (!(tt > 0) || !isfinite(tt)) {
+            rc = doci_fail(DOCI_E_BREAKDOWN, "Davidson: new direction has zero or invalid norm");
+            goto done;
+        }
+        s[m] = 1.0 / sqrt(tt);
+        for (int j = 0; j < m; ++j) G[j + (size_t)m * ld] = s[j] * s[m] * a[j];
+        G[m + (size_t)m * ld] = s[m] * s[m] * tt;
+        w.basis[m] = tslot;
+        Vp[m] = w.Vs[tslot];
+        /* W_m = (H - shift) V_m */
+        CK(timed_apply(r, w.Vs[tslot], w.Ws[tslot], prec, shift));
+        /* new column of V^T W (davidson.c:178-193) */
+        DK(dk_multidot(m + 1, Vp, w.Ws[tslot], n, prec, c->d_red, c->d_out, c->stream));
+        CK(dh_fetch(c, m + 2, red));
+        CK(add_sigma_time(r, prec));
+        for (int i = 0; i <= m; ++i)
+            S[i + (size_t)m * ld] = s[i] * s[m] * red[1 + i] + shift * G[i + (size_t)m * ld];
+        ++m;
+        CK(dh_gen_eig(m, S, G, ld, th, Y, ld));
+        if (m == mv) {
+            /* deflation (davidson.c:229-255): keep the `keep` lowest Ritz vectors */
+            int nb[DK_MAX_VEC];
+            for (int j = 0; j < m; ++j) {
+                Vp[j] = w.Vs[w.basis[j]];
+                Wp[j] = w.Ws[w.basis[j]];
+            }
+            for (int k = 0; k < keep; ++k) {
+                nb[k] = take_slot(&w);
+                for (int j = 0; j < m; ++j) g[j] = s[j] * Y[j + (size_t)k * ld];
+                DK(dk_lincomb(m, Vp, g, w.Vs[nb[k]], n, prec, prec, c->stream));
+                DK(dk_lincomb(m, Wp, g, w.Ws[nb[k]], n, prec, prec, c->stream));
+            }
+            for (int j = 0; j < m; ++j) give_slot(&w, w.basis[j]);
+            m = keep;
+            for (int k = 0; k < keep; ++k) {
+                w.basis[k] = nb[k];
+                s[k] = 1.0;
+                Vp[k] = w.Vs[nb[k]];
+            }
+            for (int k = 0; k < keep; ++k) {
+                DK(dk_multidot(k + 1, Vp, w.Vs[nb[k]], n, prec, c->d_red, c->d_out, c->stream));
+                CK(dh_fetch(c, k + 2, red));
+                G[k + (size_t)k * ld] = red[0];
+                for (int i = 0; i < k; ++i) G[i + (size_t)k * ld] = red[1 + i];
+                DK(dk_multidot(k + 1, Vp, w.Ws[nb[k]], n, prec, c->d_red, c->d_out, c->stream));
+                CK(dh_fetch(c, k + 2, red));
+                for (int i = 0; i <= k; ++i)
+                    S[i + (size_t)k * ld] = red[1 + i] + shift * G[i + (size_t)k * ld];
+            }
+            CK(dh_gen_eig(m, S, G, ld, th, Y, ld));
+        }
+        theta = th[0];
+        /* residual, preconditioner and the dots for the next orthogonalisation */
+        for (int j = 0; j < m; ++j) {
+            Vp[j] = w.Vs[w.basis[j]];
+            Wp[j] = w.Ws[w.basis[j]];
+            g[j] = s[j] * Y[j];
+        }
+        tslot = take_slot(&w);
+        DK(dk_residual(m, Vp, Wp, g, theta, shift, c->d_D, w.Vs[tslot], NULL, n, prec, c->d_red,
+                       c->d_out, c->stream));
+        CK(dh_fetch(c, m + 2, red));
+        res = sqrt(red[0]);
+        tt = red[1];
+        for (int j = 0; j < m; ++j) a[j] = red[2 + j];
+        *energy = theta;
+        if (prec) r->info->iterations_fp32++;
+        if (o->trace) printf("%d\t%d\t%e\t%f\n", *r->its, 0, res, theta);
+    }
+    /* Ritz vector x = V y (davidson.c:205) */
+    if (m > 0) {
+        for (int j = 0; j < m; ++j) {
+            Vp[j] = w.Vs[w.basis[j]];
+            g[j] = s[j] * Y[j];
+        }
+        DK(dk_lincomb(m, Vp, g, d_x, n, prec, 0, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
+    (void)start_its;
+    *res_out = res;
+    rc = DOCI_OK;
+done:
+#undef CK
+#undef DK
+    ws_free(&w);
+    free(S);
+    free(G);
+    free(Y);
+    free(th);
+    free(s);
+    free(a);
+    free(red);
+    free(g);
+    free(Vp);
+    free(Wp);
+    return rc;
+}
+
+int dh_davidson(dh_ctx *c, const doci_solve_opts *o, double *d_x, double *energy,
+                doci_solve_info *info)
+{
+    if (o->davidson_max_vec < 2 || o->davidson_max_vec > DK_MAX_VEC || o->davidson_keep < 1 ||
+        o->davidson_keep >= o->davidson_max_vec)
+        return doci_fail(DOCI_E_ARG, "need 1 <= DAVIDSON_KEEP < DAVIDSON_MAX_VEC <= %d", DK_MAX_VEC);
+    const double t0 = dh_now();
+    int its = 0, rc;
+    double res = 0;
+    dav_run r = {c, o, info, &its, NULL, NULL};
+    HIPCHK(hipEventCreate(&r.ev0));
+    HIPCHK(hipEventCreate(&r.ev1));
+    *energy = 0;
+    if (o->precision == 1 && c->fp32_ok) {
+        /* shift = min(D): fetched from the device argmin */
+        double shift = 0;
+        rc = dk_argmin(c->d_D, c->n, c->d_red, c->d_idx, c->stream);
+        int64_t idx = 0;
+        if (!rc) rc = hipMemcpyAsync(&idx, c->d_idx, sizeof idx, hipMemcpyDeviceToHost, c->stream);
+        if (!rc) rc = hipStreamSynchronize(c->stream);
+        if (!rc) rc = hipMemcpy(&shift, c->d_D + idx, sizeof shift, hipMemcpyDeviceToHost);
+        if (rc) {
+            rc = doci_fail(DOCI_E_HIP, "argmin of the diagonal failed");
+            goto out;
+        }
+        const double sw = o->switch_tol > o->tol ? o->switch_tol : o->tol;
+        rc = dav_phase(&r, 1, shift, sw, d_x, energy, &res);
+        if (rc) goto out;
+    }
+    if (res > o->tol || o->precision != 1 || !c->fp32_ok) {
+        if (its < o->max_its || its == 0) rc = dav_phase(&r, 0, 0.0, o->tol, d_x, energy, &res);
+        if (rc) goto out;
+    }
+    rc = its >= o->max_its;
+out:
+    hipEventDestroy(r.ev0);
+    hipEventDestroy(r.ev1);
+    info->iterations = its;
+    info->residual = res;
+    info->energy = *energy;
+    info->converged = res <= o->tol;
+    info->seconds = dh_now() - t0;
+    return rc;
+}

@@ -1,0 +1,94 @@
+/* internal.h -- shared declarations of the C host layer. */
+#ifndef DOCI_INTERNAL_H
+#define DOCI_INTERNAL_H
+
+#include <stdint.h>
+#include <stdio.h>
+
+#include "../../../include/doci_amd.h"
+#include "../dk.h"
+
+#define DOCI_OP_MAGIC 0x444f4349414d4431ull /* "DOCIAMD1" */
+
+struct doci_op {
+    uint64_t magic;
+    unsigned L, N;
+    int64_t n;
+    double core;
+    /* host tables (transformed integrals) */
+    double *K, *J, *X, *h;
+    uint32_t *binom;
+    double *D_host; /* lazily filled for doci_get_diagonal */
+    /* device */
+    hipStream_t stream;
+    dk_tables dt;   /* device pointers */
+    void *d_tables; /* one allocation holding the tables */
+    double *d_D;
+    double *d_red;  /* reduction scratch */
+    double *d_out;  /* small reduction results */
+    double *h_out;  /* pinned mirror of d_out */
+    int64_t *d_idx;
+    void *d_io[2];  /* staging for host-pointer sigma calls (lazy) */
+    /* scratch for shifted sigma (lazy) */
+};
+
+/* error reporting */
+int doci_fail(int code, const char *fmt, ...);
+#define HIPCHK(expr)                                                              \
+    do {                                                                          \
+        hipError_t e_ = (expr);                                                   \
+        if (e_ != hipSuccess)                                                     \
+            return doci_fail(DOCI_E_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #expr, \
+                             hipGetErrorString(e_));                              \
+    } while (0)
+#define DKCHK(expr)                                                               \
+    do {                                                                          \
+        int e_ = (expr);                                                          \
+        if (e_ != 0)                                                              \
+            return doci_fail(DOCI_E_HIP, "%s:%d %s: %s", __FILE__, __LINE__, #expr, \
+                             hipGetErrorString((hipError_t)e_));                  \
+    } while (0)
+
+/* fcidump.c */
+int dh_read_fcidump(const char *path, unsigned *L, unsigned *Na, unsigned *Nb, double *core,
+                    double **one, double **two, int *errline);
+
+/* integrals.c */
+void dh_transform_integrals(double *one, double *two, unsigned L);
+uint64_t dh_binom(unsigned k, unsigned n);
+
+/* operator.c */
+int dh_is_op(const void *p);
+
+/* dense.c: small dense eigensolvers */
+void dh_jacobi(int m, double *A, int lda, double *w, double *Z, int ldz);
+int dh_gen_eig(int m, const double *S, const double *G, int ld, double *w, double *Y, int ldy);
+int dh_sygv2(const double *A, const double *B, double *w, double *Z);
+
+/* Solver context: everything the device Davidson / CG need besides the
+ * options.  `apply` computes out = (H - shift) in on device vectors. */
+typedef struct dh_ctx {
+    int64_t n;
+    hipStream_t stream;
+    const double *d_D;
+    double *d_red, *d_out, *h_out;
+    int64_t *d_idx;
+    int (*apply)(struct dh_ctx *c, const void *in, void *out, int prec, double shift);
+    void *user;
+    int fp32_ok; /* apply supports prec 1 */
+} dh_ctx;
+
+void dh_ctx_of_op(doci_op *op, dh_ctx *c);
+int dh_fetch(dh_ctx *c, int nvals, double *host);
+
+/* davidson.c / cg.c: d_x holds the FP64 guess on entry, the unit Ritz
+ * vector on exit.  Return the reference's code (Davidson 0/1, CG 1) or a
+ * negative DOCI_E_*. */
+int dh_davidson(dh_ctx *c, const doci_solve_opts *o, double *d_x, double *energy,
+                doci_solve_info *info);
+int dh_ray_q(dh_ctx *c, const doci_solve_opts *o, int precond, double *d_x, double *energy,
+             doci_solve_info *info);
+
+double dh_now(void);
+
+#endif

@@ -1,0 +1,313 @@
+/* operator.c -- the device-backed DOCI operator (drop-in for src/doci.c:69-238).
+ *
+ * fill_doci_data builds the integral tables on the host exactly as the
+ * reference does (transform, exchange table), uploads them once, and
+ * computes the diagonal on the GPU.  doci_matvec keeps the reference's
+ * host-pointer contract; the device-resident solvers call dk_sigma directly.
+ */
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/time.h>
+
+#include "internal.h"
+
+static __thread char g_err[512];
+
+const char *doci_last_error(void) { return g_err; }
+
+int doci_fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+double dh_now(void)
+{
+    struct timeval tv;
+    gettimeofday(&tv, NULL);
+    return (double)tv.tv_sec + 1e-6 * (double)tv.tv_usec;
+}
+
+int dh_is_op(const void *p) { return p && ((const doci_op *)p)->magic == DOCI_OP_MAGIC; }
+
+void doci_op_free(doci_op *op)
+{
+    if (!op) return;
+    if (op->d_tables) hipFree(op->d_tables);
+    if (op->d_D) hipFree(op->d_D);
+    if (op->d_red) hipFree(op->d_red);
+    if (op->d_out) hipFree(op->d_out);
+    if (op->d_idx) hipFree(op->d_idx);
+    if (op->d_io[0]) hipFree(op->d_io[0]);
+    if (op->d_io[1]) hipFree(op->d_io[1]);
+    if (op->h_out) hipHostFree(op->h_out);
+    if (op->stream) hipStreamDestroy(op->stream);
+    free(op->K);
+    free(op->J);
+    free(op->X);
+    free(op->h);
+    free(op->binom);
+    free(op->D_host);
+    op->magic = 0;
+    free(op);
+}
+
+/* Takes ownership of nothing: one/two are transformed in a private copy. */
+int doci_op_from_integrals(unsigned L, unsigned N, double core, const double *one_raw,
+                           const double *two_raw, doci_op **out)
+{
+    *out = NULL;
+    if (L == 0 || L > DK_MAX_L) return doci_fail(DOCI_E_ORBITALS, "Too many orbitals (L=%u).", L);
+    if (N == 0 || N > L) return doci_fail(DOCI_E_DIM, "invalid number of pairs N=%u", N);
+    const uint64_t n = dh_binom(N, L);
+    if (n >= (1ull << 31))
+        return doci_fail(DOCI_E_DIM, "C(%u,%u) = %llu exceeds 2^31-1", L, N, (unsigned long long)n);
+    const size_t l = L, l2 = l * l, l3 = l2 * l;
+    doci_op *op = calloc(1, sizeof *op);
+    double *one = malloc(sizeof(double) * l2), *two = malloc(sizeof(double) * l2 * l2);
+    if (!op || !one || !two) {
+        free(op);
+        free(one);
+        free(two);
+        return doci_fail(DOCI_E_NOMEM, "host tables");
+    }
+    op->magic = DOCI_OP_MAGIC;
+    op->L = L;
+    op->N = N;
+    op->n = (int64_t)n;
+    op->core = core;
+    memcpy(one, one_raw, sizeof(double) * l2);
+    memcpy(two, two_raw, sizeof(double) * l2 * l2);
+    dh_transform_integrals(one, two, L);
+
+    const int N2 = (int)N + 2;
+    op->K = malloc(sizeof(double) * l2);
+    op->J = malloc(sizeof(double) * l2);
+    op->X = malloc(sizeof(double) * l2);
+    op->h = malloc(sizeof(double) * l);
+    op->binom = malloc(sizeof(uint32_t) * (l + 1) * (size_t)N2);
+    if (!op->K || !op->J || !op->X || !op->h || !op->binom) {
+        free(one);
+        free(two);
+        doci_op_free(op);
+        return doci_fail(DOCI_E_NOMEM, "host tables");
+    }
+    for (size_t p = 0; p < l; ++p) {
+        op->h[p] = one[p * (l + 1)];
+        for (size_t q = 0; q < l; ++q) {
+            op->J[p * l + q] = two[p * (l + 1) + q * (l2 + l3)];      /* (pp|qq) */
+            op->X[p * l + q] = two[p * (1 + l3) + q * (l + l2)];      /* (pq|qp) */
+            op->K[p * l + q] = p == q ? 0.0 : two[p * (l2 + 1) + q * (l3 + l)]; /* (pq|pq) */
+        }
+    }
+    for (size_t x = 0; x <= l; ++x)
+        for (int j = 0; j < N2; ++j) op->binom[x * N2 + j] = (uint32_t)dh_binom((unsigned)j, (unsigned)x);
+    free(one);
+    free(two);
+
+    /* device side */
+    int rc = DOCI_OK;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        doci_op_free(op);
+        return doci_fail(DOCI_E_HIP, "no HIP device available");
+    }
+#define TRY(expr)                                                                     \
+    do {                                                                              \
+        hipError_t e_ = (expr);                                                       \
+        if (e_ != hipSuccess) {                                                       \
+            rc = doci_fail(DOCI_E_HIP, "%s: %s", #expr, hipGetErrorString(e_));       \
+            goto fail;                                                                \
+        }                                                                             \
+    } while (0)
+    TRY(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
+    const size_t tb = sizeof(double) * (3 * l2 + l) + sizeof(uint32_t) * (l + 1) * (size_t)N2;
+    TRY(hipMalloc(&op->d_tables, tb));
+    {
+        char *b = (char *)op->d_tables;
+        double *dK = (double *)b, *dJ = dK + l2, *dX = dJ + l2, *dh = dX + l2;
+        uint32_t *db = (uint32_t *)(dh + l);
+        TRY(hipMemcpyAsync(dK, op->K, sizeof(double) * l2, hipMemcpyHostToDevice, op->stream));
+        TRY(hipMemcpyAsync(dJ, op->J, sizeof(double) * l2, hipMemcpyHostToDevice, op->stream));
+        TRY(hipMemcpyAsync(dX, op->X, sizeof(double) * l2, hipMemcpyHostToDevice, op->stream));
+        TRY(hipMemcpyAsync(dh, op->h, sizeof(double) * l, hipMemcpyHostToDevice, op->stream));
+        TRY(hipMemcpyAsync(db, op->binom, sizeof(uint32_t) * (l + 1) * (size_t)N2,
+                           hipMemcpyHostToDevice, op->stream));
+        op->dt.L = L;
+        op->dt.N = N;
+        op->dt.n = (uint32_t)n;
+        op->dt.core = core;
+        op->dt.K = dK;
+        op->dt.J = dJ;
+        op->dt.X = dX;
+        op->dt.h = dh;
+        op->dt.binom = db;
+    }
+    TRY(hipMalloc((void **)&op->d_D, sizeof(double) * n));
+    TRY(hipMalloc((void **)&op->d_red, sizeof(double) * dk_red_scratch_doubles(DK_MAX_VEC + 8)));
+    TRY(hipMalloc((void **)&op->d_out, sizeof(double) * (DK_MAX_VEC + 8)));
+    TRY(hipMalloc((void **)&op->d_idx, sizeof(int64_t)));
+    TRY(hipHostMalloc((void **)&op->h_out, sizeof(double) * (DK_MAX_VEC + 8), 0));
+    TRY((hipError_t)dk_diagonal(&op->dt, op->d_D, op->stream));
+    TRY(hipStreamSynchronize(op->stream));
+#undef TRY
+    *out = op;
+    return DOCI_OK;
+fail:
+    doci_op_free(op);
+    return rc;
+}
+
+int doci_op_from_fcidump(const char *path, doci_op **out, int *errline)
+{
+    unsigned L, Na, Nb;
+    double core, *one = NULL, *two = NULL;
+    *out = NULL;
+    int rc = dh_read_fcidump(path, &L, &Na, &Nb, &core, &one, &two, errline);
+    if (rc != DOCI_OK) return rc;
+    if (Na != Nb) {
+        free(one);
+        free(two);
+        return doci_fail(DOCI_E_SPIN,
+                         "Number of up and down electrons not equal in FCIDUMP-file.\n"
+                         "This is obligatory for DOCI.");
+    }
+    rc = doci_op_from_integrals(L, Na, core, one, two, out);
+    free(one);
+    free(two);
+    return rc;
+}
+
+int64_t doci_op_dim(const doci_op *op) { return op->n; }
+unsigned doci_op_L(const doci_op *op) { return op->L; }
+unsigned doci_op_N(const doci_op *op) { return op->N; }
+void *doci_op_stream(doci_op *op) { return (void *)op->stream; }
+
+int doci_op_diagonal(doci_op *op, double *out)
+{
+    HIPCHK(hipMemcpyAsync(out, op->d_D, sizeof(double) * op->n, hipMemcpyDeviceToHost, op->stream));
+    HIPCHK(hipStreamSynchronize(op->stream));
+    return DOCI_OK;
+}
+
+int doci_op_sigma_dev(doci_op *op, const void *in, void *out, int prec, double shift)
+{
+    if (!in || !out || in == out) return doci_fail(DOCI_E_ARG, "sigma needs distinct buffers");
+    DKCHK(dk_sigma(&op->dt, op->d_D, shift, in, out, prec, op->stream));
+    return DOCI_OK;
+}
+
+static int ensure_io(doci_op *op)
+{
+    for (int k = 0; k < 2; ++k)
+        if (!op->d_io[k]) HIPCHK(hipMalloc(&op->d_io[k], sizeof(double) * op->n));
+    return DOCI_OK;
+}
+
+int doci_op_sigma(doci_op *op, const double *in, double *out)
+{
+    int rc = ensure_io(op);
+    if (rc) return rc;
+    const size_t b = sizeof(double) * op->n;
+    HIPCHK(hipMemcpyAsync(op->d_io[0], in, b, hipMemcpyHostToDevice, op->stream));
+    DKCHK(dk_sigma(&op->dt, op->d_D, 0.0, op->d_io[0], op->d_io[1], 0, op->stream));
+    HIPCHK(hipMemcpyAsync(out, op->d_io[1], b, hipMemcpyDeviceToHost, op->stream));
+    HIPCHK(hipStreamSynchronize(op->stream));
+    return DOCI_OK;
+}
+
+/* Copies nvals reduction results from d_out to the host (synchronises). */
+int dh_fetch(dh_ctx *c, int nvals, double *host)
+{
+    HIPCHK(hipMemcpyAsync(c->h_out, c->d_out, sizeof(double) * nvals, hipMemcpyDeviceToHost,
+                          c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    memcpy(host, c->h_out, sizeof(double) * nvals);
+    return DOCI_OK;
+}
+
+static int apply_op(dh_ctx *c, const void *in, void *out, int prec, double shift)
+{
+    return doci_op_sigma_dev((doci_op *)c->user, in, out, prec, shift);
+}
+
+void dh_ctx_of_op(doci_op *op, dh_ctx *c)
+{
+    memset(c, 0, sizeof *c);
+    c->n = op->n;
+    c->stream = op->stream;
+    c->d_D = op->d_D;
+    c->d_red = op->d_red;
+    c->d_out = op->d_out;
+    c->h_out = op->h_out;
+    c->d_idx = op->d_idx;
+    c->apply = apply_op;
+    c->user = op;
+    c->fp32_ok = 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* drop-in entry points (include/doci.h)                                    */
+
+void fill_doci_data(char *dumpfil, void **dat)
+{
+    doci_op *op = NULL;
+    int line = 0;
+    const int rc = doci_op_from_fcidump(dumpfil, &op, &line);
+    if (rc != DOCI_OK) {
+        if (rc == DOCI_E_IO)
+            fprintf(stderr, "Error reading fcidump file\n");
+        else if (rc == DOCI_E_SPIN) {
+            fprintf(stderr, "%s\n", doci_last_error());
+            printf("Exiting...\n");
+        } else if (rc == DOCI_E_ORBITALS) {
+            fprintf(stderr, "Too many orbitals.\n");
+            printf("Exiting...\n");
+        } else
+            fprintf(stderr, "%s\n", doci_last_error());
+        exit(EXIT_FAILURE);
+    }
+    printf("----------\n"
+           "\t** executing DOCI (1 GPU)\n"
+           "\t** N = (%u,%u)\n"
+           "\t** L = %u\n"
+           "\t** basis size : %lld\n"
+           "----------\n",
+           op->N, op->N, op->L, (long long)op->n);
+    *dat = op;
+}
+
+double *doci_get_diagonal(void *dat)
+{
+    doci_op *op = dat;
+    if (!op->D_host) {
+        op->D_host = malloc(sizeof(double) * op->n);
+        if (!op->D_host || doci_op_diagonal(op, op->D_host) != DOCI_OK) {
+            fprintf(stderr, "doci_get_diagonal: %s\n", doci_last_error());
+            exit(EXIT_FAILURE);
+        }
+    }
+    return op->D_host;
+}
+
+int doci_get_basis_size(void *dat) { return (int)((doci_op *)dat)->n; }
+
+void doci_cleanup_data(void **dat)
+{
+    doci_op_free((doci_op *)*dat);
+    *dat = NULL;
+}
+
+void doci_matvec(double *vec, double *result, void *dat)
+{
+    if (doci_op_sigma((doci_op *)dat, vec, result) != DOCI_OK) {
+        fprintf(stderr, "doci_matvec: %s\n", doci_last_error());
+        exit(EXIT_FAILURE);
+    }
+}

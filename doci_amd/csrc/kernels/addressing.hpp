@@ -1,0 +1,84 @@
+// addressing.hpp -- device-side determinant addressing for DOCI.
+//
+// A determinant of N pairs in L orbitals is a 64-bit occupation mask (bit p =
+// orbital p).  Ranks follow the reference's lexicographic order of sorted
+// occupied tuples with the highest pair varying fastest (src/doci.c:472-548):
+//   rank(c_0 < ... < c_{N-1}) = C(L,N) - 1 - sum_e C(L-1-c_e, N-e).
+// All rank arithmetic is modulo 2^32; every true rank is < 2^31.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dk {
+
+// Binomial table layout shared by every kernel: bt[x * N2 + j] = C(x, j),
+// x in [0, L], j in [0, N+1], N2 = N + 2.
+struct Binom {
+    const uint32_t *bt;
+    int N2;
+    __device__ __forceinline__ uint32_t operator()(int x, int j) const { return bt[x * N2 + j]; }
+};
+
+// Rank -> mask by walking the lexicographic tree (O(L) table lookups).
+__device__ __forceinline__ uint64_t unrank(uint32_t r, int L, int N, const Binom &C)
+{
+    uint64_t m = 0;
+    int p = 0;
+    for (int e = 0; e < N; ++e) {
+        const int need = N - e - 1;
+        for (;;) {
+            const uint32_t blk = C(L - 1 - p, need);
+            if (r < blk) break;
+            r -= blk;
+            ++p;
+        }
+        m |= 1ull << p;
+        ++p;
+    }
+    return m;
+}
+
+// Visit every single pair hop J = I - {o} + {t} of determinant I (rank r,
+// mask m): o runs over occupied orbitals ascending, t over empty orbitals,
+// first downwards from o-1 then upwards from o+1.  rank(J) is updated in O(1)
+// per step:
+//   t < o: rank(J) = r + C(L-1-o, N-h) - C(L-1-t, N-j) + s_dn,
+//          s_dn = sum over occupied c_e in (t, o) of C(L-1-c_e, N-e) - C(L-1-c_e, N-e-1)
+//   t > o: rank(J) = r + C(L-1-o, N-h) - C(L-1-t, N-j+1) - s_up,
+//          s_up = sum over occupied c_e in (o, t) of C(L-1-c_e, N-e+1) - C(L-1-c_e, N-e)
+// where h is o's pair index and j = #{occupied orbitals of I below t}.
+template <typename F>
+__device__ __forceinline__ void for_each_hop(uint32_t r, uint64_t m, int L, int N,
+                                             const Binom &C, F &&f)
+{
+    uint64_t rem = m;
+    int h = 0;
+    while (rem) {
+        const int o = __ffsll((unsigned long long)rem) - 1;
+        rem &= rem - 1;
+        const uint32_t base = r + C(L - 1 - o, N - h);
+        int j = h;
+        uint32_t s = 0;
+        for (int t = o - 1; t >= 0; --t) {
+            if ((m >> t) & 1) {
+                --j;
+                s += C(L - 1 - t, N - j) - C(L - 1 - t, N - j - 1);
+            } else {
+                f(o, t, base - C(L - 1 - t, N - j) + s);
+            }
+        }
+        j = h + 1;
+        s = 0;
+        for (int t = o + 1; t < L; ++t) {
+            if ((m >> t) & 1) {
+                s += C(L - 1 - t, N - j + 1) - C(L - 1 - t, N - j);
+                ++j;
+            } else {
+                f(o, t, base - C(L - 1 - t, N - j + 1) - s);
+            }
+        }
+        ++h;
+    }
+}
+
+} // namespace dk

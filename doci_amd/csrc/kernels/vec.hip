@@ -1,0 +1,514 @@
+// vec.hip -- fused streaming kernels for the Davidson and CG iterations.
+//
+// Every kernel streams its vectors once with a grid-stride loop, keeps FP64
+// accumulators in registers and writes one partial per block; a second
+// launch reduces the partials in a fixed order, so every reduction is
+// deterministic for a given n (independent of scheduling).
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "../dk.h"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr int kMaxBlocks = 2048;
+
+struct Ptrs {
+    const void *p[DK_MAX_VEC];
+};
+struct Coef {
+    double g[DK_MAX_VEC];
+};
+
+inline int grid_for(int64_t n)
+{
+    int64_t b = (n + kBlock * 4 - 1) / (kBlock * 4);
+    if (b < 1) b = 1;
+    if (b > kMaxBlocks) b = kMaxBlocks;
+    return (int)b;
+}
+
+template <typename T>
+__device__ __forceinline__ double ld(const void *p, int64_t i)
+{
+    return (double)((const T *)p)[i];
+}
+
+// Reduce acc[0..nv) over the block and store one partial per value at
+// red[blockIdx.x * nv + k].
+template <int NV>
+__device__ __forceinline__ void block_partials(double (&acc)[NV], int nv, double *red)
+{
+    __shared__ double sh[NV][kWaves];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        if (k < nv) {
+            double v = acc[k];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+            if (lane == 0) sh[k][wave] = v;
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < nv; k += blockDim.x) {
+        double s = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) s += sh[k][w];
+        red[(size_t)blockIdx.x * nv + k] = s;
+    }
+}
+
+// One block per value: sum the per-block partials in a fixed tree order.
+__global__ __launch_bounds__(kBlock) void k_finalize(const double *__restrict__ red, int nblocks,
+                                                     int nv, double *__restrict__ out)
+{
+    __shared__ double sh[kBlock];
+    const int k = blockIdx.x;
+    double s = 0;
+    for (int b = threadIdx.x; b < nblocks; b += kBlock) s += red[(size_t)b * nv + k];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = sh[0];
+}
+
+int finalize(const double *red, int nblocks, int nv, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(nv), dim3(kBlock), 0, s, red, nblocks, nv, out);
+    return (int)hipGetLastError();
+}
+
+// ---- multidot: out[0] = <y,y>, out[1+j] = <X_j, y> -------------------------
+template <typename T, int M>
+__global__ __launch_bounds__(kBlock) void k_multidot(Ptrs X, int m, const T *__restrict__ y,
+                                                     int64_t n, double *__restrict__ red)
+{
+    double acc[M + 1];
+#pragma unroll
+    for (int k = 0; k <= M; ++k) acc[k] = 0;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        const double yi = (double)y[i];
+        acc[0] += yi * yi;
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+            if (j < m) acc[1 + j] += ld<T>(X.p[j], i) * yi;
+    }
+    block_partials<M + 1>(acc, m + 1, red);
+}
+
+// ---- ortho: y -= sum g_j X_j; out[0] = <y,y>, out[1+j] = <X_j,y> -----------
+template <typename T, int M>
+__global__ __launch_bounds__(kBlock) void k_ortho(Ptrs X, int m, Coef g, T *__restrict__ y,
+                                                  int64_t n, double *__restrict__ red)
+{
+    double acc[M + 1];
+#pragma unroll
+    for (int k = 0; k <= M; ++k) acc[k] = 0;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        double v[M];
+        double yv = (double)y[i];
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+            if (j < m) {
+                v[j] = ld<T>(X.p[j], i);
+                yv -= g.g[j] * v[j];
+            }
+        const T yq = (T)yv;
+        y[i] = yq;
+        const double yd = (double)yq;
+        acc[0] += yd * yd;
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+            if (j < m) acc[1 + j] += v[j] * yd;
+    }
+    block_partials<M + 1>(acc, m + 1, red);
+}
+
+// ---- Davidson residual (src/davidson.c:195-227 fused) ---------------------
+// out[0] = <r,r>, out[1] = <t,t>, out[2+j] = <V_j,t>
+template <typename T, int M>
+__global__ __launch_bounds__(kBlock) void k_residual(Ptrs V, Ptrs W, int m, Coef a, double theta,
+                                                     double shift, const double *__restrict__ D,
+                                                     T *__restrict__ t, T *__restrict__ xo,
+                                                     int64_t n, double *__restrict__ red)
+{
+    double acc[M + 2];
+#pragma unroll
+    for (int k = 0; k < M + 2; ++k) acc[k] = 0;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        double v[M];
+        double x = 0, w = 0;
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+            if (j < m) {
+                v[j] = ld<T>(V.p[j], i);
+                x += a.g[j] * v[j];
+                w += a.g[j] * ld<T>(W.p[j], i);
+            }
+        const double r = w - (theta - shift) * x;
+        const double d = fabs(D[i] - theta);
+        const double tv = d > 1e-12 ? r / d : r / 1e-12;
+        const T tq = (T)tv;
+        t[i] = tq;
+        if (xo) xo[i] = (T)x;
+        const double td = (double)tq;
+        acc[0] += r * r;
+        acc[1] += td * td;
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+            if (j < m) acc[2 + j] += v[j] * td;
+    }
+    block_partials<M + 2>(acc, m + 2, red);
+}
+
+template <typename T, typename To, int M>
+__global__ __launch_bounds__(kBlock) void k_lincomb(Ptrs X, int m, Coef g, To *__restrict__ out,
+                                                    int64_t n)
+{
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        double s = 0;
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+            if (j < m) s += g.g[j] * ld<T>(X.p[j], i);
+        out[i] = (To)s;
+    }
+}
+
+template <typename Ti, typename To>
+__global__ __launch_bounds__(kBlock) void k_convert(const Ti *__restrict__ in, To *__restrict__ out,
+                                                    int64_t n, double scale)
+{
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+        out[i] = (To)(scale * (double)in[i]);
+}
+
+// ---- CG (src/CG.c) ---------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ void cg_grad_elem(double x, double ax, double rho, const double *pinv,
+                                             int64_t i, T *g, double (&acc)[2])
+{
+    double gv = 2 * (ax + (-rho) * x);
+    acc[0] += gv * gv;
+    if (pinv) gv *= pinv[i];
+    const T gq = (T)gv;
+    g[i] = gq;
+    acc[1] += (double)gq * (double)gq;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_grad(const T *__restrict__ x, const T *__restrict__ Ax,
+                                                    double rho, const double *__restrict__ pinv,
+                                                    T *__restrict__ g, int64_t n,
+                                                    double *__restrict__ red)
+{
+    double acc[2] = {0, 0};
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+        cg_grad_elem<T>((double)x[i], (double)Ax[i], rho, pinv, i, g, acc);
+    block_partials<2>(acc, 2, red);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_step(T *__restrict__ x, const T *__restrict__ p,
+                                                    T *__restrict__ Ax, const T *__restrict__ Ap,
+                                                    double z0, double z1, double rho,
+                                                    const double *__restrict__ pinv,
+                                                    T *__restrict__ g, int64_t n,
+                                                    double *__restrict__ red)
+{
+    double acc[2] = {0, 0};
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        const T xn = (T)(z0 * (double)x[i] + z1 * (double)p[i]);
+        const T axn = (T)(z0 * (double)Ax[i] + z1 * (double)Ap[i]);
+        x[i] = xn;
+        Ax[i] = axn;
+        cg_grad_elem<T>((double)xn, (double)axn, rho, pinv, i, g, acc);
+    }
+    block_partials<2>(acc, 2, red);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_dir(T *__restrict__ p, const T *__restrict__ g,
+                                                   double q, int first, int64_t n)
+{
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+        p[i] = first ? (T)(-(double)g[i]) : (T)(q * (double)p[i] + (-1.0) * (double)g[i]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_cg_gram(const T *__restrict__ x, const T *__restrict__ p,
+                                                    const T *__restrict__ Ax,
+                                                    const T *__restrict__ Ap, int64_t n,
+                                                    double *__restrict__ red)
+{
+    double acc[7] = {0, 0, 0, 0, 0, 0, 0};
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+        const double xv = (double)x[i], pv = (double)p[i], ax = (double)Ax[i], ap = (double)Ap[i];
+        acc[0] += xv * ax;
+        acc[1] += pv * ax;
+        acc[2] += xv * ap;
+        acc[3] += pv * ap;
+        acc[4] += xv * xv;
+        acc[5] += pv * xv;
+        acc[6] += pv * pv;
+    }
+    block_partials<7>(acc, 7, red);
+}
+
+__global__ __launch_bounds__(kBlock) void k_precond_inv(const double *__restrict__ D,
+                                                        double *__restrict__ pinv, int64_t n)
+{
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+        pinv[i] = fabs(D[i]) > 1e-12 ? 1 / D[i] : 1 / 1e-12;
+}
+
+// First index of the minimum: (value, index) pairs, ties to the lower index.
+__device__ __forceinline__ void amin_merge(double &v, double &ix, double v2, double ix2)
+{
+    if (v2 < v || (v2 == v && ix2 < ix)) {
+        v = v2;
+        ix = ix2;
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_argmin_part(const double *__restrict__ D, int64_t n,
+                                                        double *__restrict__ red)
+{
+    __shared__ double sv[kBlock], si[kBlock];
+    double v = INFINITY, ix = 1e300;
+    const int64_t stride = (int64_t)gridDim.x * kBlock;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
+        amin_merge(v, ix, D[i], (double)i);
+    sv[threadIdx.x] = v;
+    si[threadIdx.x] = ix;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            double a = sv[threadIdx.x], b = si[threadIdx.x];
+            amin_merge(a, b, sv[threadIdx.x + w], si[threadIdx.x + w]);
+            sv[threadIdx.x] = a;
+            si[threadIdx.x] = b;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        red[2 * blockIdx.x] = sv[0];
+        red[2 * blockIdx.x + 1] = si[0];
+    }
+}
+
+__global__ void k_argmin_fin(const double *__restrict__ red, int nb, int64_t *__restrict__ idx)
+{
+    if (threadIdx.x == 0) {
+        double v = INFINITY, ix = 1e300;
+        for (int b = 0; b < nb; ++b) amin_merge(v, ix, red[2 * b], red[2 * b + 1]);
+        *idx = (int64_t)ix;
+    }
+}
+
+} // namespace
+
+#define DK_BUCKET(m, CALL)                                                       \
+    do {                                                                         \
+        if ((m) <= 8) { constexpr int MM = 8; CALL; }                            \
+        else if ((m) <= 16) { constexpr int MM = 16; CALL; }                     \
+        else if ((m) <= 32) { constexpr int MM = 32; CALL; }                     \
+        else { constexpr int MM = 64; CALL; }                                    \
+    } while (0)
+
+static Ptrs make_ptrs(int m, void *const *X)
+{
+    Ptrs p;
+    for (int j = 0; j < DK_MAX_VEC; ++j) p.p[j] = j < m ? X[j] : nullptr;
+    return p;
+}
+static Coef make_coef(int m, const double *g)
+{
+    Coef c;
+    for (int j = 0; j < DK_MAX_VEC; ++j) c.g[j] = j < m ? g[j] : 0.0;
+    return c;
+}
+
+extern "C" size_t dk_red_scratch_doubles(int nvals)
+{
+    return (size_t)kMaxBlocks * (size_t)(nvals < 2 ? 2 : nvals);
+}
+
+extern "C" int dk_multidot(int m, void *const *X, const void *y, int64_t n, int prec,
+                           double *red, double *out, hipStream_t s)
+{
+    if (m < 0 || m > DK_MAX_VEC) return (int)hipErrorInvalidValue;
+    const int nb = grid_for(n);
+    const Ptrs P = make_ptrs(m, X);
+    if (prec == 0)
+        DK_BUCKET(m, hipLaunchKernelGGL((k_multidot<double, MM>), dim3(nb), dim3(kBlock), 0, s, P, m,
+                                        (const double *)y, n, red));
+    else
+        DK_BUCKET(m, hipLaunchKernelGGL((k_multidot<float, MM>), dim3(nb), dim3(kBlock), 0, s, P, m,
+                                        (const float *)y, n, red));
+    int e = (int)hipGetLastError();
+    return e ? e : finalize(red, nb, m + 1, out, s);
+}
+
+extern "C" int dk_ortho(int m, void *const *X, const double *g, void *y, int64_t n, int prec,
+                        double *red, double *out, hipStream_t s)
+{
+    if (m < 0 || m > DK_MAX_VEC) return (int)hipErrorInvalidValue;
+    const int nb = grid_for(n);
+    const Ptrs P = make_ptrs(m, X);
+    const Coef G = make_coef(m, g);
+    if (prec == 0)
+        DK_BUCKET(m, hipLaunchKernelGGL((k_ortho<double, MM>), dim3(nb), dim3(kBlock), 0, s, P, m, G,
+                                        (double *)y, n, red));
+    else
+        DK_BUCKET(m, hipLaunchKernelGGL((k_ortho<float, MM>), dim3(nb), dim3(kBlock), 0, s, P, m, G,
+                                        (float *)y, n, red));
+    int e = (int)hipGetLastError();
+    return e ? e : finalize(red, nb, m + 1, out, s);
+}
+
+extern "C" int dk_residual(int m, void *const *V, void *const *W, const double *a, double theta,
+                           double shift, const double *D, void *t, void *x_out, int64_t n, int prec,
+                           double *red, double *out, hipStream_t s)
+{
+    if (m < 1 || m > DK_MAX_VEC) return (int)hipErrorInvalidValue;
+    const int nb = grid_for(n);
+    const Ptrs PV = make_ptrs(m, V), PW = make_ptrs(m, W);
+    const Coef A = make_coef(m, a);
+    if (prec == 0)
+        DK_BUCKET(m, hipLaunchKernelGGL((k_residual<double, MM>), dim3(nb), dim3(kBlock), 0, s, PV, PW,
+                                        m, A, theta, shift, D, (double *)t, (double *)x_out, n, red));
+    else
+        DK_BUCKET(m, hipLaunchKernelGGL((k_residual<float, MM>), dim3(nb), dim3(kBlock), 0, s, PV, PW,
+                                        m, A, theta, shift, D, (float *)t, (float *)x_out, n, red));
+    int e = (int)hipGetLastError();
+    return e ? e : finalize(red, nb, m + 2, out, s);
+}
+
+extern "C" int dk_lincomb(int m, void *const *X, const double *g, void *out, int64_t n, int prec,
+                          int prec_out, hipStream_t s)
+{
+    if (m < 1 || m > DK_MAX_VEC) return (int)hipErrorInvalidValue;
+    const int nb = grid_for(n);
+    const Ptrs P = make_ptrs(m, X);
+    const Coef G = make_coef(m, g);
+    if (prec == 0 && prec_out == 0)
+        DK_BUCKET(m, hipLaunchKernelGGL((k_lincomb<double, double, MM>), dim3(nb), dim3(kBlock), 0, s,
+                                        P, m, G, (double *)out, n));
+    else if (prec == 0)
+        DK_BUCKET(m, hipLaunchKernelGGL((k_lincomb<double, float, MM>), dim3(nb), dim3(kBlock), 0, s,
+                                        P, m, G, (float *)out, n));
+    else if (prec_out == 0)
+        DK_BUCKET(m, hipLaunchKernelGGL((k_lincomb<float, double, MM>), dim3(nb), dim3(kBlock), 0, s,
+                                        P, m, G, (double *)out, n));
+    else
+        DK_BUCKET(m, hipLaunchKernelGGL((k_lincomb<float, float, MM>), dim3(nb), dim3(kBlock), 0, s,
+                                        P, m, G, (float *)out, n));
+    return (int)hipGetLastError();
+}
+
+extern "C" int dk_convert(const void *in, int pin, void *out, int pout, int64_t n, double scale,
+                          hipStream_t s)
+{
+    const int nb = grid_for(n);
+    if (pin == 0 && pout == 0)
+        hipLaunchKernelGGL((k_convert<double, double>), dim3(nb), dim3(kBlock), 0, s,
+                           (const double *)in, (double *)out, n, scale);
+    else if (pin == 0 && pout == 1)
+        hipLaunchKernelGGL((k_convert<double, float>), dim3(nb), dim3(kBlock), 0, s,
+                           (const double *)in, (float *)out, n, scale);
+    else if (pin == 1 && pout == 0)
+        hipLaunchKernelGGL((k_convert<float, double>), dim3(nb), dim3(kBlock), 0, s,
+                           (const float *)in, (double *)out, n, scale);
+    else
+        hipLaunchKernelGGL((k_convert<float, float>), dim3(nb), dim3(kBlock), 0, s,
+                           (const float *)in, (float *)out, n, scale);
+    return (int)hipGetLastError();
+}
+
+extern "C" int dk_cg_grad(const void *x, const void *Ax, double rho, const double *pinv, void *g,
+                          int64_t n, int prec, double *red, double *out, hipStream_t s)
+{
+    const int nb = grid_for(n);
+    if (prec == 0)
+        hipLaunchKernelGGL(k_cg_grad<double>, dim3(nb), dim3(kBlock), 0, s, (const double *)x,
+                           (const double *)Ax, rho, pinv, (double *)g, n, red);
+    else
+        hipLaunchKernelGGL(k_cg_grad<float>, dim3(nb), dim3(kBlock), 0, s, (const float *)x,
+                           (const float *)Ax, rho, pinv, (float *)g, n, red);
+    int e = (int)hipGetLastError();
+    return e ? e : finalize(red, nb, 2, out, s);
+}
+
+extern "C" int dk_cg_dir(void *p, const void *g, double q, int first, int64_t n, int prec,
+                         hipStream_t s)
+{
+    const int nb = grid_for(n);
+    if (prec == 0)
+        hipLaunchKernelGGL(k_cg_dir<double>, dim3(nb), dim3(kBlock), 0, s, (double *)p,
+                           (const double *)g, q, first, n);
+    else
+        hipLaunchKernelGGL(k_cg_dir<float>, dim3(nb), dim3(kBlock), 0, s, (float *)p,
+                           (const float *)g, q, first, n);
+    return (int)hipGetLastError();
+}
+
+extern "C" int dk_cg_gram(const void *x, const void *p, const void *Ax, const void *Ap, int64_t n,
+                          int prec, double *red, double *out, hipStream_t s)
+{
+    const int nb = grid_for(n);
+    if (prec == 0)
+        hipLaunchKernelGGL(k_cg_gram<double>, dim3(nb), dim3(kBlock), 0, s, (const double *)x,
+                           (const double *)p, (const double *)Ax, (const double *)Ap, n, red);
+    else
+        hipLaunchKernelGGL(k_cg_gram<float>, dim3(nb), dim3(kBlock), 0, s, (const float *)x,
+                           (const float *)p, (const float *)Ax, (const float *)Ap, n, red);
+    int e = (int)hipGetLastError();
+    return e ? e : finalize(red, nb, 7, out, s);
+}
+
+extern "C" int dk_cg_step(void *x, const void *p, void *Ax, const void *Ap, double z0, double z1,
+                          double rho, const double *pinv, void *g, int64_t n, int prec,
+                          double *red, double *out, hipStream_t s)
+{
+    const int nb = grid_for(n);
+    if (prec == 0)
+        hipLaunchKernelGGL(k_cg_step<double>, dim3(nb), dim3(kBlock), 0, s, (double *)x,
+                           (const double *)p, (double *)Ax, (const double *)Ap, z0, z1, rho, pinv,
+                           (double *)g, n, red);
+    else
+        hipLaunchKernelGGL(k_cg_step<float>, dim3(nb), dim3(kBlock), 0, s, (float *)x,
+                           (const float *)p, (float *)Ax, (const float *)Ap, z0, z1, rho, pinv,
+                           (float *)g, n, red);
+    int e = (int)hipGetLastError();
+    return e ? e : finalize(red, nb, 2, out, s);
+}
+
+extern "C" int dk_precond_inv(const double *D, double *pinv, int64_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_precond_inv, dim3(grid_for(n)), dim3(kBlock), 0, s, D, pinv, n);
+    return (int)hipGetLastError();
+}
+
+extern "C" int dk_argmin(const double *D, int64_t n, double *red, int64_t *idx, hipStream_t s)
+{
+    const int nb = grid_for(n);
+    hipLaunchKernelGGL(k_argmin_part, dim3(nb), dim3(kBlock), 0, s, D, n, red);
+    hipLaunchKernelGGL(k_argmin_fin, dim3(1), dim3(64), 0, s, red, nb, idx);
+    return (int)hipGetLastError();
+}

@@ -1,0 +1,196 @@
+/*
+ * doci_amd.h -- public C ABI of libdoci_amd.so, a DOCI ground-state solver
+ * whose diagonal, sigma = H c and Davidson / CG iterations run resident in
+ * the HBM of one AMD Instinct MI355X.
+ *
+ * Part 1 is the drop-in surface: the same names, signatures, argument
+ * meaning and error behaviour (print to stderr, exit(EXIT_FAILURE)) as the
+ * reference program klgunst/doci.  Part 2 is this library's own
+ * device-resident API (doci_*), which returns error codes instead of exiting.
+ */
+#ifndef DOCI_AMD_H
+#define DOCI_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ======================================================================== */
+/* Part 1: drop-in entry points                                             */
+/* ======================================================================== */
+
+/* Replaces include/read_fcidump.h:4 (src/read_fcidump.c:18-82).  *one_p_int
+ * (L*L) and *two_p_int (L^4) are malloc'd; the caller frees them. */
+void read_fcidump(char *filename, unsigned int *NORB, unsigned int *NALPHA,
+                  unsigned int *NBETA, double *core_energy, double **one_p_int,
+                  double **two_p_int);
+
+/* Replaces include/doci.h:14 (src/doci.c:69-134).  Reads the FCIDUMP, builds
+ * the integral and addressing tables on the GPU and computes the diagonal
+ * there.  Limits: Na == Nb, L <= 64 (64-bit masks), C(L,N) < 2^31. */
+void fill_doci_data(char *dumpfil, void **dat);
+
+/* Replaces include/doci.h:12 (src/doci.c:136-140): host copy of the diagonal,
+ * owned by dat (freed by doci_cleanup_data). */
+double *doci_get_diagonal(void *dat);
+
+/* Replaces include/doci.h:16 (src/doci.c:142-146). */
+int doci_get_basis_size(void *dat);
+
+/* Replaces include/doci.h:18 (src/doci.c:148-159); sets *dat = NULL. */
+void doci_cleanup_data(void **dat);
+
+/* Replaces include/doci.h:31 (src/doci.c:164-238): result = H vec for host
+ * vectors of doci_get_basis_size(dat) doubles (copied through the GPU). */
+void doci_matvec(double *vec, double *result, void *dat);
+
+/* Replaces include/wrapper_solvers.h:36-38 (src/wrapper_solvers.c:10-24).
+ * solver "D" (Davidson), "CG", "CGP" (CG with 1/diag preconditioner);
+ * returns -1 for an unknown solver, Davidson 0/1 (converged / max_its hit),
+ * CG and CGP 1.  When matvec == doci_matvec and dat came from
+ * fill_doci_data the whole solve runs resident on the GPU and only vec and
+ * *energy cross back; any other matvec is called on host vectors while the
+ * solver's vector algebra runs on the GPU. */
+int sparse_eigensolve(double *vec, int dims, double *energy,
+                      void (*matvec)(double *, double *, void *), double *diagonal,
+                      double *M, double tol, int max_its, char solver[],
+                      int davidson_keep, int davidson_max_vec, void *dat);
+
+/* Replaces include/davidson.h:25-27 (src/davidson.c:33-128). */
+int davidson(double *result, double *energy, int max_vectors, int keep_deflate,
+             double davidson_tol, void (*matvec)(double *, double *, void *),
+             double *diagonal, int basis_size, int max_its, void *vdat);
+
+/* Replaces include/CG.h:4-5 (src/CG.c:13-164): (matvec, M, vec, size, energy,
+ * tol, max_its, precond, dat).  M must be NULL. */
+int ray_q(void (*matvec)(double *, double *, void *), double *M, double *vec,
+          int size, double *energy, double cg_tol, int max_its, double *precond,
+          void *vdat);
+
+/* ======================================================================== */
+/* Part 2: device-resident API                                              */
+/* ======================================================================== */
+
+enum {
+    DOCI_OK = 0,
+    DOCI_E_IO = -1,          /* file cannot be opened */
+    DOCI_E_FORMAT = -2,      /* FCIDUMP header malformed */
+    DOCI_E_LINE = -3,        /* FCIDUMP integral line malformed (see errline) */
+    DOCI_E_SPIN = -4,        /* Na != Nb */
+    DOCI_E_ORBITALS = -5,    /* L > 64 */
+    DOCI_E_DIM = -6,         /* C(L,N) >= 2^31 or N = 0 */
+    DOCI_E_HIP = -7,         /* HIP runtime error / no GPU */
+    DOCI_E_NOMEM = -8,       /* host or device allocation failed */
+    DOCI_E_ARG = -9,         /* invalid argument */
+    DOCI_E_SOLVER = -10,     /* unknown solver name */
+    DOCI_E_BREAKDOWN = -11   /* subspace breakdown (zero or non-finite norm) */
+};
+
+typedef struct doci_op doci_op;
+
+/* Last error message of the calling thread (static storage). */
+const char *doci_last_error(void);
+
+/* Operator from an FCIDUMP file or from raw integrals in the reference's
+ * layout (one[(j-1)L+(i-1)] for "v i j 0 0", two[(l-1)L^3+(k-1)L^2+(j-1)L+(i-1)]
+ * for "v i j k l"), which are transformed exactly as the reference does.
+ * *errline receives the offending line for DOCI_E_LINE (may be NULL). */
+int doci_op_from_fcidump(const char *path, doci_op **op, int *errline);
+int doci_op_from_integrals(unsigned L, unsigned N, double core, const double *one,
+                           const double *two, doci_op **op);
+void doci_op_free(doci_op *op);
+int64_t doci_op_dim(const doci_op *op);
+unsigned doci_op_L(const doci_op *op);
+unsigned doci_op_N(const doci_op *op);
+/* Copies the diagonal (bit-identical to the reference's) into host memory. */
+int doci_op_diagonal(doci_op *op, double *out);
+/* Host-buffer sigma: out = H in, FP64. */
+int doci_op_sigma(doci_op *op, const double *in, double *out);
+/* Device-buffer sigma on the operator's stream.  prec 0: FP64 vectors;
+ * prec 1: FP32 vectors with FP64 sums.  out = (H - shift) in. */
+int doci_op_sigma_dev(doci_op *op, const void *in, void *out, int prec, double shift);
+/* The operator's HIP stream (hipStream_t) and device. */
+void *doci_op_stream(doci_op *op);
+
+/* Solver settings; doci_solve_defaults gives the reference's defaults
+ * (include/options.h:4-7; src/executable.c:212-218). */
+typedef struct {
+    char solver[8];        /* "D", "CG", "CGP" */
+    double tol;            /* 1e-8 */
+    int max_its;           /* 200 */
+    int davidson_keep;     /* 2 */
+    int davidson_max_vec;  /* 30 */
+    int precision;         /* 0 = FP64, 1 = mixed (FP32 storage + FP64 finish) */
+    double switch_tol;     /* mixed: residual at which FP64 takes over (1e-5) */
+    int trace;             /* print "its INFO residue energy" lines */
+} doci_solve_opts;
+
+typedef struct {
+    int iterations;        /* total iterations (all phases) */
+    int iterations_fp32;   /* of which in FP32 storage */
+    int sigma_calls;
+    int converged;
+    double residual;       /* final residual norm (Davidson) or |2r|^2 (CG) */
+    double energy;
+    double seconds;        /* wall time inside the solve */
+    double seconds_sigma;  /* of which in sigma kernels (event-timed) */
+    int sigma_calls_fp32;  /* sigma launches on FP32 vectors */
+    double seconds_sigma_fp32; /* their event-timed total */
+} doci_solve_info;
+
+void doci_solve_defaults(doci_solve_opts *o);
+
+/* Ground state.  guess: host vector of dim doubles, or NULL for the HF guess
+ * (unit vector at the first minimum of the diagonal).  x_out (may be NULL)
+ * receives the unit eigenvector.  Returns the reference's solver code
+ * (0/1 for Davidson, 1 for CG) or a negative DOCI_E_* error. */
+int doci_solve(doci_op *op, const doci_solve_opts *o, const double *guess,
+               double *x_out, double *energy, doci_solve_info *info);
+
+/* ---- FCIDUMP reader/writer and synthetic integrals ---------------------- */
+
+/* read_fcidump's parser returning DOCI_E_IO / DOCI_E_FORMAT / DOCI_E_LINE
+ * (with *errline) instead of exiting; free the tables with doci_free. */
+int doci_read_fcidump(const char *path, unsigned *L, unsigned *Na, unsigned *Nb,
+                      double *core, double **one, double **two, int *errline);
+void doci_free(void *p);
+
+/* Writes the canonical FCIDUMP (i>=j, k>=l, ij>=kl; "%.17e"; core line last;
+ * no trailing blank line; entries with |v| < 1e-12 dropped). */
+int doci_write_fcidump(const char *path, unsigned L, unsigned N, double core,
+                       const double *one, const double *two);
+
+/* Seeded synthetic integrals in the raw layout above (canonical entries
+ * only, |v| < 1e-12 dropped): h_ii = -12 + 11 i/(L-1) + N(0,0.05),
+ * h_ij ~ N(0, 0.1 e^{-|i-j|/4}), (ij|kl) = sum_{P=1..2L} B^P_ij B^P_kl with
+ * B^P_ij ~ N(0, 0.3 e^{-|i-j|/3}/sqrt(P)); core = 10.  splitmix64 -> 53-bit
+ * uniform -> Box-Muller, drawn in the order h_ii, h_ij (i<j), B^P (i<=j). */
+int doci_gen_integrals(unsigned L, uint64_t seed, double *core, double *one,
+                       double *two);
+
+/* N(0,1) vector of n entries from the same generator, normalised in FP64. */
+int doci_gen_vector(int64_t n, uint64_t seed, double *out);
+
+/* ---- input file (src/executable.c:201-268) ------------------------------ */
+
+typedef struct {
+    char fcidump[255];
+    char solver[20];
+    int max_its;
+    double tol;
+    int davidson_keep;
+    int davidson_max_vec;
+    int hf_init;
+    int precision;         /* DOCI_PRECISION = MIXED -> 1 (a key of this library) */
+} doci_input;
+
+/* Reads the reference's key = value input file with its defaults. */
+int doci_read_inputfile(const char *path, doci_input *in);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
