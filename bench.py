@@ -61,6 +61,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--calib", action="store_true",
+                    help="first run a 1 GiB device copy (byte-counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the DOCI solve here is a single-GPU "
@@ -68,6 +70,19 @@ def main() -> int:
         return 2
     import numpy as np
     import doci_amd as da
+
+    if args.calib:
+        import ctypes
+        hip = ctypes.CDLL("libamdhip64.so")
+        nb = 1 << 30
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(nb)) == 0
+        assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(nb)) == 0
+        assert hip.hipMemset(a, 0, ctypes.c_size_t(nb)) == 0
+        da._check(da.lib().doci_dev_copy(b, a, nb // 8), "doci_dev_copy")
+        hip.hipFree(a)
+        hip.hipFree(b)
+        log("calibration copy: 1 GiB read + 1 GiB written by k_convert<double,double>")
 
     log(f"operator L={L} N={N} seed={SEED}")
     t0 = time.perf_counter()

@@ -56,7 +56,7 @@ EXPORTS = [
     "doci_op_dim", "doci_op_L", "doci_op_N", "doci_op_diagonal", "doci_op_sigma",
     "doci_op_sigma_dev", "doci_op_stream", "doci_solve_defaults", "doci_solve",
     "doci_write_fcidump", "doci_gen_integrals", "doci_gen_vector", "doci_read_inputfile",
-    "doci_read_fcidump", "doci_free",
+    "doci_read_fcidump", "doci_free", "doci_dev_copy",
 ]
 
 
@@ -96,6 +96,7 @@ def lib():
                                     C.POINTER(C.POINTER(C.c_double)),
                                     C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.c_int)]
     l.doci_free.argtypes = [vp]
+    l.doci_dev_copy.argtypes = [vp, vp, C.c_int64]
     _lib = l
     return l
 

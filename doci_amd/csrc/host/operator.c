@@ -189,6 +189,13 @@ unsigned doci_op_L(const doci_op *op) { return op->L; }
 unsigned doci_op_N(const doci_op *op) { return op->N; }
 void *doci_op_stream(doci_op *op) { return (void *)op->stream; }
 
+int doci_dev_copy(void *dst, const void *src, int64_t n)
+{
+    DKCHK(dk_convert(src, 0, dst, 0, n, 1.0, 0));
+    HIPCHK(hipDeviceSynchronize());
+    return DOCI_OK;
+}
+
 int doci_op_diagonal(doci_op *op, double *out)
 {
     HIPCHK(hipMemcpyAsync(out, op->d_D, sizeof(double) * op->n, hipMemcpyDeviceToHost, op->stream));

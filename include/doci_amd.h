@@ -115,6 +115,10 @@ int doci_op_sigma_dev(doci_op *op, const void *in, void *out, int prec, double s
 /* The operator's HIP stream (hipStream_t) and device. */
 void *doci_op_stream(doci_op *op);
 
+/* Plain device copy of n doubles through this library's streaming kernel
+ * (synchronous); used to calibrate hardware byte counters. */
+int doci_dev_copy(void *dst, const void *src, int64_t n);
+
 /* Solver settings; doci_solve_defaults gives the reference's defaults
  * (include/options.h:4-7; src/executable.c:212-218). */
 typedef struct {
