@@ -157,7 +157,7 @@ static int dav_phase(dav_run *r, int prec, double shift, double stop_tol, double
     DK(dk_multidot(0, NULL, w.Vs[tslot], n, prec, c->d_red, c->d_out, c->stream));
     CK(dh_fetch(c, 1, red));
     tt = red[0];
-    double res = o->tol * 10, theta = 0;
+    double res = stop_tol * 10, theta = 0; /* davidson.c:45: start above the tolerance */
     const int start_its = *r->its;
     while (res > stop_tol && (*r->its)++ < o->max_its) {
         /* orthogonalise t against V (davidson.c:133-176) */
