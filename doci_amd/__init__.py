@@ -56,7 +56,8 @@ EXPORTS = [
     "doci_op_dim", "doci_op_L", "doci_op_N", "doci_op_diagonal", "doci_op_sigma",
     "doci_op_sigma_dev", "doci_op_stream", "doci_solve_defaults", "doci_solve",
     "doci_write_fcidump", "doci_gen_integrals", "doci_gen_vector", "doci_read_inputfile",
-    "doci_read_fcidump", "doci_free", "doci_dev_copy",
+    "doci_read_fcidump", "doci_free", "doci_dev_copy", "doci_op_set_kernel",
+    "doci_op_slab_split", "doci_op_check_violations",
 ]
 
 
@@ -97,6 +98,11 @@ def lib():
                                     C.POINTER(C.POINTER(C.c_double)), C.POINTER(C.c_int)]
     l.doci_free.argtypes = [vp]
     l.doci_dev_copy.argtypes = [vp, vp, C.c_int64]
+    l.doci_op_set_kernel.argtypes = [vp, C.c_int]
+    l.doci_op_slab_split.argtypes = [vp, C.POINTER(C.c_uint), C.POINTER(C.c_uint),
+                                     C.POINTER(C.c_uint)]
+    l.doci_op_check_violations.argtypes = [vp]
+    l.doci_op_check_violations.restype = C.c_int64
     _lib = l
     return l
 
@@ -233,6 +239,21 @@ class Operator:
     def sigma_dev(self, d_in: int, d_out: int, prec: int = 0, shift: float = 0.0) -> None:
         """Device pointers (ints), on the operator's stream."""
         _check(lib().doci_op_sigma_dev(self._h, d_in, d_out, prec, shift), "doci_op_sigma_dev")
+
+    KERNELS = {"slab": 0, "lane": 1, "slab-checked": 2}
+
+    def set_kernel(self, name: str) -> None:
+        """Sigma kernel: "slab" (default), "lane" or "slab-checked"."""
+        _check(lib().doci_op_set_kernel(self._h, self.KERNELS[name]), "doci_op_set_kernel")
+
+    def slab_split(self):
+        a, b, g = C.c_uint(), C.c_uint(), C.c_uint()
+        _check(lib().doci_op_slab_split(self._h, C.byref(a), C.byref(b), C.byref(g)),
+               "doci_op_slab_split")
+        return a.value, b.value, g.value
+
+    def check_violations(self) -> int:
+        return _check(lib().doci_op_check_violations(self._h), "doci_op_check_violations")
 
     def stream(self) -> int:
         return int(lib().doci_op_stream(self._h) or 0)

@@ -39,6 +39,30 @@ int dk_diagonal(const dk_tables *t, double *D, hipStream_t s);
 int dk_sigma(const dk_tables *t, const double *D, double shift, const void *c,
              void *sigma, int prec, hipStream_t s);
 
+/* Slab decomposition for the slab-blocked sigma: B = the top b orbitals, A
+ * the other a = L - b.  Slab i holds the C(b, m) determinants whose A-part is
+ * slab_amask[i], starting at rank slab_start[i].  Group g = slabs
+ * [group[2g], group[2g] + (group[2g+1] & 0xffff)) which all have
+ * m = group[2g+1] >> 16 pairs in B and together at most emax determinants
+ * (at most gmax slabs).  aamax/abmax/bamax bound the per-slab neighbour
+ * tables: max k(a-k), max k and max a-k over the groups' k = N - m. */
+typedef struct {
+    uint32_t a, b, ngroup, emax, gmax, aamax, abmax, bamax;
+    const uint32_t *slab_start;
+    const uint64_t *slab_amask;
+    const uint32_t *group;
+} dk_slabs;
+
+/* Dynamic LDS bytes the slab kernel needs (prec as for dk_sigma). */
+size_t dk_sigma_slab_lds(const dk_tables *t, const dk_slabs *sl, int prec);
+/* sigma = (H - shift) c with the slab-blocked kernel. */
+int dk_sigma_slab(const dk_tables *t, const dk_slabs *sl, const double *D, double shift,
+                  const void *c, void *sigma, int prec, hipStream_t s);
+/* The same with every computed index bounds-checked: violations are counted
+ * in *bad (device) and the access skipped. */
+int dk_sigma_slab_checked(const dk_tables *t, const dk_slabs *sl, const double *D, double shift,
+                          const void *c, void *sigma, int prec, uint32_t *bad, hipStream_t s);
+
 /* ---- streaming vector kernels (deterministic two-level reductions) -------
  * Vectors are arrays of n elements; `vecs` lists device pointers.  prec as
  * above for the stored vectors; reductions and coefficients are FP64.

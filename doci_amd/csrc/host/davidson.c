@@ -39,11 +39,12 @@ typedef struct {
     int *basis;      /* slot index of V_j */
     int *freelist;
     int nfree;
+    int pooled;      /* block belongs to the operator's persistent pool */
 } dav_ws;
 
 static void ws_free(dav_ws *w)
 {
-    if (w->block) hipFree(w->block);
+    if (w->block && !w->pooled) hipFree(w->block);
     free(w->Vs);
     free(w->Ws);
     free(w->basis);
@@ -51,7 +52,7 @@ static void ws_free(dav_ws *w)
     memset(w, 0, sizeof *w);
 }
 
-static int ws_alloc(dav_ws *w, int64_t n, int prec, int nslot)
+static int ws_alloc(dav_ws *w, dh_ctx *c, int64_t n, int prec, int nslot)
 {
     memset(w, 0, sizeof *w);
     w->prec = prec;
@@ -66,7 +67,24 @@ static int ws_alloc(dav_ws *w, int64_t n, int prec, int nslot)
         ws_free(w);
         return doci_fail(DOCI_E_NOMEM, "Davidson host workspace");
     }
-    hipError_t e = hipMalloc(&w->block, vb * 2 * (size_t)nslot);
+    const size_t need = vb * 2 * (size_t)nslot;
+    hipError_t e = hipSuccess;
+    if (c->pool && *c->pool && *c->pool_bytes >= need) {
+        w->block = *c->pool; /* reuse the operator's subspace memory */
+        w->pooled = 1;
+    } else if (c->pool) {
+        if (*c->pool) hipFree(*c->pool);
+        *c->pool = NULL;
+        *c->pool_bytes = 0;
+        e = hipMalloc(&w->block, need);
+        if (e == hipSuccess) {
+            *c->pool = w->block;
+            *c->pool_bytes = need;
+            w->pooled = 1;
+        }
+    } else {
+        e = hipMalloc(&w->block, need);
+    }
     if (e != hipSuccess) {
         w->block = NULL;
         ws_free(w);
@@ -124,7 +142,7 @@ static int dav_phase(dav_run *r, int prec, double shift, double stop_tol, double
     const int64_t n = c->n;
     const double ortho_tol = prec ? 1e-6 : 1e-12;
     dav_ws w;
-    int rc = ws_alloc(&w, n, prec, mv + keep);
+    int rc = ws_alloc(&w, c, n, prec, mv + keep);
     if (rc) return rc;
     double *S = calloc((size_t)mv * mv, sizeof(double)), *G = calloc((size_t)mv * mv, sizeof(double));
     double *Y = calloc((size_t)mv * mv, sizeof(double)), *th = calloc((size_t)mv, sizeof(double));

@@ -29,6 +29,10 @@ struct doci_op {
     double *h_out;  /* pinned mirror of d_out */
     int64_t *d_idx;
     void *d_io[2];  /* staging for host-pointer sigma calls (lazy) */
+    void *pool;        /* Davidson subspace kept between solves (grows, freed with op) */
+    size_t pool_bytes;
+    int sigma_kind;    /* 0 = slab kernel (default), 1 = lane-per-determinant kernel */
+    void *slabs;       /* slab-kernel tables (slabs.c) */
     /* scratch for shifted sigma (lazy) */
 };
 
@@ -76,6 +80,8 @@ typedef struct dh_ctx {
     int (*apply)(struct dh_ctx *c, const void *in, void *out, int prec, double shift);
     void *user;
     int fp32_ok; /* apply supports prec 1 */
+    void **pool;        /* optional persistent workspace (owned by an operator) */
+    size_t *pool_bytes;
 } dh_ctx;
 
 void dh_ctx_of_op(doci_op *op, dh_ctx *c);
@@ -90,5 +96,11 @@ int dh_ray_q(dh_ctx *c, const doci_solve_opts *o, int precond, double *d_x, doub
              doci_solve_info *info);
 
 double dh_now(void);
+
+/* slabs.c */
+int dh_slabs_build(doci_op *op);
+void dh_slabs_free(doci_op *op);
+int dh_slabs_sigma(doci_op *op, const void *in, void *out, int prec, double shift);
+void dh_slabs_info(const doci_op *op, unsigned *a, unsigned *b, unsigned *ngroup);
 
 #endif

@@ -46,6 +46,8 @@ void doci_op_free(doci_op *op)
     if (op->d_idx) hipFree(op->d_idx);
     if (op->d_io[0]) hipFree(op->d_io[0]);
     if (op->d_io[1]) hipFree(op->d_io[1]);
+    if (op->pool) hipFree(op->pool);
+    dh_slabs_free(op);
     if (op->h_out) hipHostFree(op->h_out);
     if (op->stream) hipStreamDestroy(op->stream);
     free(op->K);
@@ -157,6 +159,8 @@ int doci_op_from_integrals(unsigned L, unsigned N, double core, const double *on
     TRY((hipError_t)dk_diagonal(&op->dt, op->d_D, op->stream));
     TRY(hipStreamSynchronize(op->stream));
 #undef TRY
+    rc = dh_slabs_build(op);
+    if (rc != DOCI_OK) goto fail;
     *out = op;
     return DOCI_OK;
 fail:
@@ -206,8 +210,24 @@ int doci_op_diagonal(doci_op *op, double *out)
 int doci_op_sigma_dev(doci_op *op, const void *in, void *out, int prec, double shift)
 {
     if (!in || !out || in == out) return doci_fail(DOCI_E_ARG, "sigma needs distinct buffers");
+    if ((op->sigma_kind == 0 || op->sigma_kind == 2) && op->slabs)
+        return dh_slabs_sigma(op, in, out, prec, shift);
     DKCHK(dk_sigma(&op->dt, op->d_D, shift, in, out, prec, op->stream));
     return DOCI_OK;
+}
+
+int doci_op_set_kernel(doci_op *op, int kind)
+{
+    if (kind < 0 || kind > 2) return doci_fail(DOCI_E_ARG, "kernel kind must be 0, 1 or 2");
+    if (kind != 1 && !op->slabs) return doci_fail(DOCI_E_ARG, "slab tables unavailable");
+    op->sigma_kind = kind;
+    return DOCI_OK;
+}
+
+int doci_op_slab_split(const doci_op *op, unsigned *a, unsigned *b, unsigned *ngroups)
+{
+    dh_slabs_info(op, a, b, ngroups);
+    return op->slabs ? DOCI_OK : doci_fail(DOCI_E_ARG, "no slab tables");
 }
 
 static int ensure_io(doci_op *op)
@@ -223,7 +243,8 @@ int doci_op_sigma(doci_op *op, const double *in, double *out)
     if (rc) return rc;
     const size_t b = sizeof(double) * op->n;
     HIPCHK(hipMemcpyAsync(op->d_io[0], in, b, hipMemcpyHostToDevice, op->stream));
-    DKCHK(dk_sigma(&op->dt, op->d_D, 0.0, op->d_io[0], op->d_io[1], 0, op->stream));
+    rc = doci_op_sigma_dev(op, op->d_io[0], op->d_io[1], 0, 0.0);
+    if (rc) return rc;
     HIPCHK(hipMemcpyAsync(out, op->d_io[1], b, hipMemcpyDeviceToHost, op->stream));
     HIPCHK(hipStreamSynchronize(op->stream));
     return DOCI_OK;
@@ -257,6 +278,8 @@ void dh_ctx_of_op(doci_op *op, dh_ctx *c)
     c->apply = apply_op;
     c->user = op;
     c->fp32_ok = 1;
+    c->pool = &op->pool;
+    c->pool_bytes = &op->pool_bytes;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -115,6 +115,15 @@ int doci_op_sigma_dev(doci_op *op, const void *in, void *out, int prec, double s
 /* The operator's HIP stream (hipStream_t) and device. */
 void *doci_op_stream(doci_op *op);
 
+/* Sigma kernel of the operator: 0 = slab-blocked (default), 1 = one lane per
+ * determinant, 2 = slab-blocked with every computed index bounds-checked
+ * (violations counted, see doci_op_check_violations, and the access skipped). */
+int doci_op_set_kernel(doci_op *op, int kind);
+/* The slab split: a low orbitals, b high orbitals, number of workgroups. */
+int doci_op_slab_split(const doci_op *op, unsigned *a, unsigned *b, unsigned *ngroups);
+/* Index violations counted by kernel kind 2 since the operator was built. */
+int64_t doci_op_check_violations(doci_op *op);
+
 /* Plain device copy of n doubles through this library's streaming kernel
  * (synchronous); used to calibrate hardware byte counters. */
 int doci_dev_copy(void *dst, const void *src, int64_t n);

@@ -23,6 +23,65 @@ def rel_err(a, b):
     return np.abs(a - b).max() / np.abs(b).max()
 
 
+def _checked_case(op, seed=3):
+    """Bounds-checked slab kernel first (no violation allowed), then the
+    unchecked slab and the lane kernel must agree with it."""
+    c = da.gen_vector(op.n, seed)
+    op.set_kernel("slab-checked")
+    s_chk = op.sigma(c)
+    assert op.check_violations() == 0
+    op.set_kernel("slab")
+    s_slab = op.sigma(c)
+    op.set_kernel("lane")
+    s_lane = op.sigma(c)
+    op.set_kernel("slab")
+    assert np.array_equal(s_chk, s_slab)
+    assert rel_err(s_slab, s_lane) <= SIGMA_RTOL
+    return c, s_slab
+
+
+@pytest.mark.parametrize("name", SMALL + LARGE + ["syn", "L34"])
+def test_slab_kernel_bounds_then_parity(name, fcidump_dir, syn24, tmp_path_factory):
+    """Runs before every other sigma test: the slab kernel's index arithmetic
+    is proven in bounds on each shape used later (including L=34, N=10)."""
+    print(f"[case] slab checked {name}", flush=True)
+    if name == "L34":
+        op = da.Operator.synthetic(34, 10, 2)
+    elif name == "syn":
+        op = da.Operator.from_fcidump(str(syn24))
+    else:
+        op = da.Operator.from_fcidump(str(fcidump_dir / f"{name}.FCIDUMP"))
+    with op:
+        a, b, g = op.slab_split()
+        assert a + b == op.L and g > 0
+        _checked_case(op)
+        # FP32 vectors through the checked kernel as well
+        op.set_kernel("slab-checked")
+        _sigma_fp32(op, da.gen_vector(op.n, 5))
+        assert op.check_violations() == 0
+        op.set_kernel("slab")
+
+
+def _sigma_fp32(op, c):
+    import ctypes
+    n = op.n
+    hip = ctypes.CDLL("libamdhip64.so")
+    c32 = c.astype(np.float32)
+    din, dout = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(din), ctypes.c_size_t(4 * n)) == 0
+    assert hip.hipMalloc(ctypes.byref(dout), ctypes.c_size_t(4 * n)) == 0
+    try:
+        assert hip.hipMemcpy(din, c32.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(4 * n), 1) == 0
+        op.sigma_dev(din.value, dout.value, prec=1)
+        out = np.zeros(n, dtype=np.float32)
+        assert hip.hipDeviceSynchronize() == 0
+        assert hip.hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), dout, ctypes.c_size_t(4 * n), 2) == 0
+    finally:
+        hip.hipFree(din)
+        hip.hipFree(dout)
+    return out
+
+
 @pytest.mark.parametrize("name", SMALL + LARGE)
 def test_diagonal_bit_exact(name, fcidump_dir, golden):
     print(f"[case] diagonal {name}", flush=True)
@@ -215,27 +274,14 @@ def test_mixed_precision_matches_fp64(fcidump_dir, syn24):
 def test_sigma_fp32_storage(fcidump_dir):
     """FP32 vectors with FP64 sums: error at FP32 rounding of the output."""
     print("[case] sigma fp32", flush=True)
-    import ctypes
     path = str(fcidump_dir / "lif_631g_1.FCIDUMP")
-    hip = ctypes.CDLL("libamdhip64.so")
     with da.Operator.from_fcidump(path) as op:
-        n = op.n
-        c = da.gen_vector(n, 3)
+        c = da.gen_vector(op.n, 3)
         ref = op.sigma(c)
-        c32 = c.astype(np.float32)
-        din, dout = ctypes.c_void_p(), ctypes.c_void_p()
-        assert hip.hipMalloc(ctypes.byref(din), ctypes.c_size_t(4 * n)) == 0
-        assert hip.hipMalloc(ctypes.byref(dout), ctypes.c_size_t(4 * n)) == 0
-        try:
-            assert hip.hipMemcpy(din, c32.ctypes.data_as(ctypes.c_void_p), ctypes.c_size_t(4 * n), 1) == 0
-            op.sigma_dev(din.value, dout.value, prec=1)
-            out = np.zeros(n, dtype=np.float32)
-            assert hip.hipDeviceSynchronize() == 0
-            assert hip.hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), dout, ctypes.c_size_t(4 * n), 2) == 0
-        finally:
-            hip.hipFree(din)
-            hip.hipFree(dout)
-    assert rel_err(out.astype(np.float64), ref) < 3e-7
+        for kern in ("slab", "lane"):
+            op.set_kernel(kern)
+            out = _sigma_fp32(op, c)
+            assert rel_err(out.astype(np.float64), ref) < 3e-7
 
 
 def test_dropin_c_entry_points(fcidump_dir, golden):
