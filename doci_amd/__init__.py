@@ -57,7 +57,8 @@ EXPORTS = [
     "doci_op_sigma_dev", "doci_op_stream", "doci_solve_defaults", "doci_solve",
     "doci_write_fcidump", "doci_gen_integrals", "doci_gen_vector", "doci_read_inputfile",
     "doci_read_fcidump", "doci_free", "doci_dev_copy", "doci_op_set_kernel",
-    "doci_op_slab_split", "doci_op_check_violations",
+    "doci_op_slab_split", "doci_op_check_violations", "doci_op_tune", "doci_op_time_sigma",
+    "doci_op_star_split",
 ]
 
 
@@ -103,6 +104,11 @@ def lib():
                                      C.POINTER(C.c_uint)]
     l.doci_op_check_violations.argtypes = [vp]
     l.doci_op_check_violations.restype = C.c_int64
+    l.doci_op_tune.argtypes = [vp, C.c_char_p, C.c_int]
+    l.doci_op_star_split.argtypes = [vp, C.POINTER(C.c_uint), C.POINTER(C.c_uint),
+                                     C.POINTER(C.c_uint), C.POINTER(C.c_uint),
+                                     C.POINTER(C.c_size_t)]
+    l.doci_op_time_sigma.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
     _lib = l
     return l
 
@@ -240,10 +246,10 @@ class Operator:
         """Device pointers (ints), on the operator's stream."""
         _check(lib().doci_op_sigma_dev(self._h, d_in, d_out, prec, shift), "doci_op_sigma_dev")
 
-    KERNELS = {"slab": 0, "lane": 1, "slab-checked": 2}
+    KERNELS = {"star": 0, "lane": 1, "star-checked": 2, "slab": 3, "slab-checked": 4}
 
     def set_kernel(self, name: str) -> None:
-        """Sigma kernel: "slab" (default), "lane" or "slab-checked"."""
+        """Sigma kernel: "star" (default), "lane", "slab", or a "-checked" variant."""
         _check(lib().doci_op_set_kernel(self._h, self.KERNELS[name]), "doci_op_set_kernel")
 
     def slab_split(self):
@@ -251,6 +257,21 @@ class Operator:
         _check(lib().doci_op_slab_split(self._h, C.byref(a), C.byref(b), C.byref(g)),
                "doci_op_slab_split")
         return a.value, b.value, g.value
+
+    def tune(self, key: str, value: int) -> None:
+        _check(lib().doci_op_tune(self._h, key.encode(), int(value)), f"doci_op_tune({key})")
+
+    def time_sigma(self, prec: int = 0, reps: int = 5) -> float:
+        """Average sigma duration in ms (HIP events on the operator's stream)."""
+        ms = C.c_double()
+        _check(lib().doci_op_time_sigma(self._h, prec, reps, C.byref(ms)), "doci_op_time_sigma")
+        return ms.value
+
+    def star_split(self):
+        a, b, g, sg, z = C.c_uint(), C.c_uint(), C.c_uint(), C.c_uint(), C.c_size_t()
+        _check(lib().doci_op_star_split(self._h, C.byref(a), C.byref(b), C.byref(g),
+                                        C.byref(sg), C.byref(z)), "doci_op_star_split")
+        return dict(a=a.value, b=b.value, groups=g.value, star_groups=sg.value, zslots=z.value)
 
     def check_violations(self) -> int:
         return _check(lib().doci_op_check_violations(self._h), "doci_op_check_violations")

@@ -48,6 +48,7 @@ int dk_sigma(const dk_tables *t, const double *D, double shift, const void *c,
  * tables: max k(a-k), max k and max a-k over the groups' k = N - m. */
 typedef struct {
     uint32_t a, b, ngroup, emax, gmax, aamax, abmax, bamax;
+    uint32_t xcd_runs;       /* 1: deal contiguous runs of groups to each XCD; 0: in order */
     const uint32_t *slab_start;
     const uint64_t *slab_amask;
     const uint32_t *group;
@@ -62,6 +63,44 @@ int dk_sigma_slab(const dk_tables *t, const dk_slabs *sl, const double *D, doubl
  * in *bad (device) and the access skipped. */
 int dk_sigma_slab_checked(const dk_tables *t, const dk_slabs *sl, const double *D, double shift,
                           const void *c, void *sigma, int prec, uint32_t *bad, hipStream_t s);
+
+/* Star decomposition for the two-pass sigma (B = top b orbitals, A the
+ * other a).  A slab is the set of C(b, m) determinants sharing an A-part
+ * alpha (k = N - m pairs in A).  The star of rho (rho in A, j pairs) is the
+ * centre slab rho (m = N - j pairs in B; absent when N - j > b) with the
+ * member slabs rho + y, y an empty A orbital.
+ * Star pass: one workgroup stages whole stars in LDS and writes, into the
+ * slot buffer Z, for every member rho + x the sum of its hops that empty x
+ * (to another A orbital: dense K_Y products; to a B orbital: the centre), and
+ * for the centre the sum of its hops from B into A (the members).
+ * Main pass: sigma = (D - shift) c + hops inside the slab + sum of the slab's
+ * slots.  Slab alpha (k pairs in A) has nslot = k + [m >= 1 && k < a] slots
+ * of C(b, m) values starting at slab_zbase; member slot = position of x in
+ * alpha, centre slot = k.  zoff[k] = first Z index of the k-pair sector, whose
+ * slabs are stored in lexicographic order of alpha. */
+typedef struct {
+    uint32_t a, b;
+    uint32_t ngroup;            /* main-pass groups */
+    uint32_t emax;              /* determinants staged per main-pass group */
+    uint32_t nsgroup;           /* star-pass groups */
+    uint32_t sgmax;             /* stars per star group, at most */
+    uint32_t selems;            /* LDS elements staged per star group, at most */
+    uint32_t ymax;              /* largest number of members of a star */
+    const uint32_t *slab_start; /* per slab (order: k, then lexicographic alpha) */
+    const uint32_t *slab_zbase;
+    const uint32_t *group;      /* main groups: first slab, nslab | m << 16 */
+    const uint64_t *star_amask; /* stars (order: j, then lexicographic rho) */
+    const uint32_t *sgroup;     /* star groups: first star, nstar | j << 16 */
+    const uint32_t *zoff;       /* [a + 2]: first Z index of each k-sector */
+} dk_stars;
+
+/* LDS bytes of the star pass and of the main pass (prec as for dk_sigma). */
+size_t dk_sigma_star_lds(const dk_tables *t, const dk_stars *st, int prec, int pass);
+/* sigma = (H - shift) c by the star pass then the main pass.  Z: slot buffer
+ * of zoff[a+1] elements of the vector precision.  bad: NULL, or a device
+ * counter for the bounds-checked instantiation. */
+int dk_sigma_star(const dk_tables *t, const dk_stars *st, const double *D, double shift,
+                  const void *c, void *sigma, void *Z, int prec, uint32_t *bad, hipStream_t s);
 
 /* ---- streaming vector kernels (deterministic two-level reductions) -------
  * Vectors are arrays of n elements; `vecs` lists device pointers.  prec as

@@ -31,8 +31,9 @@ struct doci_op {
     void *d_io[2];  /* staging for host-pointer sigma calls (lazy) */
     void *pool;        /* Davidson subspace kept between solves (grows, freed with op) */
     size_t pool_bytes;
-    int sigma_kind;    /* 0 = slab kernel (default), 1 = lane-per-determinant kernel */
+    int sigma_kind;    /* 0 star (default), 1 lane, 2 star checked, 3 slab, 4 slab checked */
     void *slabs;       /* slab-kernel tables (slabs.c) */
+    void *stars;       /* star-kernel tables (stars.c) */
     /* scratch for shifted sigma (lazy) */
 };
 
@@ -98,9 +99,18 @@ int dh_ray_q(dh_ctx *c, const doci_solve_opts *o, int precond, double *d_x, doub
 double dh_now(void);
 
 /* slabs.c */
-int dh_slabs_build(doci_op *op);
+int dh_slabs_build(doci_op *op, unsigned emax, unsigned gmax, int xcd_runs);
 void dh_slabs_free(doci_op *op);
 int dh_slabs_sigma(doci_op *op, const void *in, void *out, int prec, double shift);
 void dh_slabs_info(const doci_op *op, unsigned *a, unsigned *b, unsigned *ngroup);
+int64_t dh_slabs_violations(doci_op *op);
+
+/* stars.c */
+int dh_stars_build(doci_op *op);
+void dh_stars_free(doci_op *op);
+int dh_stars_sigma(doci_op *op, const void *in, void *out, int prec, double shift, int checked);
+int64_t dh_stars_violations(doci_op *op);
+void dh_stars_info(const doci_op *op, unsigned *a, unsigned *b, unsigned *ngroup,
+                   unsigned *nsgroup, size_t *zelems);
 
 #endif

@@ -48,6 +48,7 @@ void doci_op_free(doci_op *op)
     if (op->d_io[1]) hipFree(op->d_io[1]);
     if (op->pool) hipFree(op->pool);
     dh_slabs_free(op);
+    dh_stars_free(op);
     if (op->h_out) hipHostFree(op->h_out);
     if (op->stream) hipStreamDestroy(op->stream);
     free(op->K);
@@ -159,7 +160,8 @@ int doci_op_from_integrals(unsigned L, unsigned N, double core, const double *on
     TRY((hipError_t)dk_diagonal(&op->dt, op->d_D, op->stream));
     TRY(hipStreamSynchronize(op->stream));
 #undef TRY
-    rc = dh_slabs_build(op);
+    rc = dh_slabs_build(op, 0, 0, 0);
+    if (rc == DOCI_OK) rc = dh_stars_build(op);
     if (rc != DOCI_OK) goto fail;
     *out = op;
     return DOCI_OK;
@@ -210,18 +212,39 @@ int doci_op_diagonal(doci_op *op, double *out)
 int doci_op_sigma_dev(doci_op *op, const void *in, void *out, int prec, double shift)
 {
     if (!in || !out || in == out) return doci_fail(DOCI_E_ARG, "sigma needs distinct buffers");
-    if ((op->sigma_kind == 0 || op->sigma_kind == 2) && op->slabs)
-        return dh_slabs_sigma(op, in, out, prec, shift);
+    switch (op->sigma_kind) {
+    case 0:
+    case 2: return dh_stars_sigma(op, in, out, prec, shift, op->sigma_kind == 2);
+    case 3:
+    case 4: return dh_slabs_sigma(op, in, out, prec, shift);
+    default: break;
+    }
     DKCHK(dk_sigma(&op->dt, op->d_D, shift, in, out, prec, op->stream));
     return DOCI_OK;
 }
 
 int doci_op_set_kernel(doci_op *op, int kind)
 {
-    if (kind < 0 || kind > 2) return doci_fail(DOCI_E_ARG, "kernel kind must be 0, 1 or 2");
-    if (kind != 1 && !op->slabs) return doci_fail(DOCI_E_ARG, "slab tables unavailable");
+    if (kind < 0 || kind > 4) return doci_fail(DOCI_E_ARG, "kernel kind must be 0..4");
+    if ((kind == 0 || kind == 2) && !op->stars) return doci_fail(DOCI_E_ARG, "no star tables");
+    if (kind >= 3 && !op->slabs) return doci_fail(DOCI_E_ARG, "slab tables unavailable");
     op->sigma_kind = kind;
     return DOCI_OK;
+}
+
+int64_t doci_op_check_violations(doci_op *op)
+{
+    const int64_t a = dh_stars_violations(op);
+    if (a < 0) return a;
+    const int64_t b = dh_slabs_violations(op);
+    return b < 0 ? b : a + b;
+}
+
+int doci_op_star_split(const doci_op *op, unsigned *a, unsigned *b, unsigned *ngroups,
+                       unsigned *nstargroups, size_t *zslots)
+{
+    dh_stars_info(op, a, b, ngroups, nstargroups, zslots);
+    return op->stars ? DOCI_OK : doci_fail(DOCI_E_ARG, "no star tables");
 }
 
 int doci_op_slab_split(const doci_op *op, unsigned *a, unsigned *b, unsigned *ngroups)

@@ -42,14 +42,15 @@ void dh_slabs_free(doci_op *op)
     op->slabs = NULL;
 }
 
-int dh_slabs_build(doci_op *op)
+int dh_slabs_build(doci_op *op, unsigned emax_req, unsigned gmax, int xcd_runs)
 {
     const unsigned L = op->L, N = op->N;
     const uint64_t n = (uint64_t)op->n;
-    unsigned emax = SLAB_EMAX;
+    unsigned emax = emax_req ? emax_req : SLAB_EMAX;
+    if (!gmax) gmax = SLAB_GMAX;
     /* LDS: K table, binomials, FP64 staging, neighbour tables */
     const size_t fixed = 8 * (size_t)L * L + 4 * (size_t)(L + 1) * (N + 2) + 64;
-    while (emax > 256 && fixed + 8 * (size_t)emax + SLAB_GMAX * (size_t)(L * L / 4 + 2 * L) * 6 >
+    while (emax > 256 && fixed + 8 * (size_t)emax + gmax * (size_t)(L * L / 4 + 2 * L) * 6 >
                              SLAB_LDS_BUDGET)
         emax /= 2;
     unsigned b = L;
@@ -97,7 +98,7 @@ int dh_slabs_build(doci_op *op)
                 msk |= 1ull << idx[e];
                 sa += dh_binom(N - e, L - 1 - (unsigned)idx[e]);
             }
-            if (gcount == SLAB_GMAX || (gcount + 1) * S > emax) {
+            if (gcount == gmax || (gcount + 1) * S > emax) {
                 grp[ng].first = gfirst;
                 grp[ng].start = start[gfirst];
                 grp[ng].word = gcount | (m << 16);
@@ -158,7 +159,8 @@ int dh_slabs_build(doci_op *op)
     s->d.b = b;
     s->d.ngroup = (uint32_t)ng;
     s->d.emax = emax;
-    s->d.gmax = SLAB_GMAX;
+    s->d.gmax = gmax;
+    s->d.xcd_runs = (uint32_t)xcd_runs;
     s->d.aamax = aamax;
     s->d.abmax = abmax;
     s->d.bamax = bamax;
@@ -194,15 +196,63 @@ int dh_slabs_sigma(doci_op *op, const void *in, void *out, int prec, double shif
 {
     dh_slabs *s = op->slabs;
     DKCHK(dk_sigma_slab_checked(&op->dt, &s->d, op->d_D, shift, in, out, prec,
-                                op->sigma_kind == 2 ? s->d_bad : NULL, op->stream));
+                                op->sigma_kind == 4 ? s->d_bad : NULL, op->stream));
     return DOCI_OK;
 }
 
-int64_t doci_op_check_violations(doci_op *op)
+int doci_op_tune(doci_op *op, const char *key, int value)
+{
+    dh_slabs *s = op->slabs;
+    if (!s) return doci_fail(DOCI_E_ARG, "no slab tables");
+    if (!strcmp(key, "xcd_runs")) {
+        s->d.xcd_runs = value != 0;
+        return DOCI_OK;
+    }
+    if (!strcmp(key, "emax") || !strcmp(key, "gmax")) {
+        const unsigned emax = !strcmp(key, "emax") ? (unsigned)value : s->d.emax;
+        const unsigned gmax = !strcmp(key, "gmax") ? (unsigned)value : s->d.gmax;
+        const int runs = (int)s->d.xcd_runs;
+        if (emax < 16 || gmax < 1 || gmax > 0xffff) return doci_fail(DOCI_E_ARG, "bad %s", key);
+        dh_slabs_free(op);
+        return dh_slabs_build(op, emax, gmax, runs);
+    }
+    return doci_fail(DOCI_E_ARG, "unknown tuning key %s", key);
+}
+
+int doci_op_time_sigma(doci_op *op, int prec, int reps, double *ms)
+{
+    const size_t es = prec ? sizeof(float) : sizeof(double);
+    void *x = NULL, *y = NULL;
+    hipEvent_t e0, e1;
+    int rc = DOCI_OK;
+    if (reps < 1) return doci_fail(DOCI_E_ARG, "reps");
+    HIPCHK(hipMalloc(&x, es * op->n));
+    HIPCHK(hipMalloc(&y, es * op->n));
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    HIPCHK(hipMemsetAsync(x, 0, es * op->n, op->stream));
+    rc = doci_op_sigma_dev(op, x, y, prec, 0.0); /* warm-up */
+    if (!rc) HIPCHK(hipEventRecord(e0, op->stream));
+    for (int r = 0; !rc && r < reps; ++r) rc = doci_op_sigma_dev(op, x, y, prec, 0.0);
+    if (!rc) {
+        float t = 0;
+        HIPCHK(hipEventRecord(e1, op->stream));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventElapsedTime(&t, e0, e1));
+        *ms = t / reps;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    hipFree(x);
+    hipFree(y);
+    return rc;
+}
+
+int64_t dh_slabs_violations(doci_op *op)
 {
     dh_slabs *s = op->slabs;
     uint32_t v = 0;
-    if (!s) return doci_fail(DOCI_E_ARG, "no slab tables");
+    if (!s) return 0;
     HIPCHK(hipStreamSynchronize(op->stream));
     HIPCHK(hipMemcpy(&v, s->d_bad, sizeof v, hipMemcpyDeviceToHost));
     return (int64_t)v;

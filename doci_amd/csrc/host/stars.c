@@ -1,0 +1,304 @@
+/* stars.c -- tables of the two-pass (star / main) sigma.
+ *
+ * b is the largest number of top orbitals (<= 16) for which the biggest star
+ * (a - j member slabs of C(b, N-j-1) plus the centre slab of C(b, N-j)) fits
+ * the star pass's LDS with FP64 staging; a larger b means fewer A pairs per
+ * determinant and so fewer Z slots to write and read.  Slabs are stored by
+ * k (pairs in A), then in lexicographic order of their A-part, so a slab's
+ * Z offset follows from its index in the sector.  Main-pass groups pack
+ * whole slabs of one sector and are ordered by rank; star groups pack whole
+ * stars of one j. */
+#include <stdlib.h>
+#include <string.h>
+
+#include "internal.h"
+
+#define STAR_LDS_BUDGET (64 * 1024)
+#define STAR_BMAX 16
+#define STAR_SGMAX 64
+#define MAIN_EMAX 4096
+
+typedef struct {
+    dk_stars d;
+    void *dmem;
+    uint32_t *d_bad;
+    void *Z;        /* slot buffer, FP64-sized so both precisions fit */
+    size_t zelems;
+} dh_stars;
+
+typedef struct {
+    uint32_t start, first, word;
+} sgrp_t;
+
+static int cmp_sgrp(const void *x, const void *y)
+{
+    const sgrp_t *a = x, *b = y;
+    return a->start < b->start ? -1 : a->start > b->start;
+}
+
+static int valid_sector(unsigned k, unsigned N, unsigned a, unsigned b)
+{
+    return k <= N && k <= a && N - k <= b;
+}
+
+static unsigned nslot(unsigned k, unsigned m, unsigned a) { return k + ((m >= 1 && k < a) ? 1 : 0); }
+
+static size_t star_lds_bytes(unsigned L, unsigned N, unsigned sgmax, unsigned ymax, unsigned selems)
+{
+#define A8(x) (((size_t)(x) + 7) & ~(size_t)7)
+    return A8(8 * (size_t)L * L) + A8(4 * (size_t)(L + 1) * (N + 2)) + A8(8 * (size_t)selems) +
+           2 * A8(4 * (size_t)sgmax * ymax) + 2 * A8(4 * (size_t)sgmax) + 2 * A8((size_t)sgmax * ymax);
+#undef A8
+}
+
+void dh_stars_free(doci_op *op)
+{
+    dh_stars *s = op->stars;
+    if (!s) return;
+    if (s->dmem) hipFree(s->dmem);
+    if (s->d_bad) hipFree(s->d_bad);
+    if (s->Z) hipFree(s->Z);
+    free(s);
+    op->stars = NULL;
+}
+
+/* lexicographic enumeration of the k-subsets of {0..a-1} */
+static int next_comb(int *idx, unsigned k, unsigned a)
+{
+    int e = (int)k - 1;
+    while (e >= 0 && idx[e] == (int)(a - k) + e) --e;
+    if (e < 0) return 0;
+    ++idx[e];
+    for (int f = e + 1; f < (int)k; ++f) idx[f] = idx[f - 1] + 1;
+    return 1;
+}
+
+int dh_stars_build(doci_op *op)
+{
+    const unsigned L = op->L, N = op->N;
+    const uint64_t n = (uint64_t)op->n;
+    unsigned b = L < STAR_BMAX ? L : STAR_BMAX, a = 0, ymax = 0, selems = 0;
+    for (; b >= 1; --b) {
+        a = L - b;
+        uint64_t mx = 0;
+        ymax = 1;
+        for (unsigned j = 0; j < N; ++j) {
+            if (!valid_sector(j + 1, N, a, b)) continue; /* members' sector */
+            const uint64_t st = (uint64_t)(a - j) * dh_binom(N - j - 1, b) +
+                                (valid_sector(j, N, a, b) ? dh_binom(N - j, b) : 0);
+            if (st > mx) mx = st;
+            if (a - j > ymax) ymax = a - j;
+        }
+        uint64_t maxS = 0; /* biggest slab: the main pass stages whole slabs */
+        for (unsigned k = 0; k <= N; ++k)
+            if (valid_sector(k, N, a, b) && dh_binom(N - k, b) > maxS) maxS = dh_binom(N - k, b);
+        if (maxS < MAIN_EMAX) maxS = MAIN_EMAX;
+        if (ymax > 64) continue;
+        if (star_lds_bytes(L, N, STAR_SGMAX, ymax, (unsigned)mx) <= STAR_LDS_BUDGET &&
+            8 * (size_t)L * L + 4 * (size_t)(L + 1) * (N + 2) + 8 * (size_t)maxS + 64 <=
+                STAR_LDS_BUDGET) {
+            /* the rest of the budget stages as many stars as fit */
+            selems = (unsigned)((STAR_LDS_BUDGET - star_lds_bytes(L, N, STAR_SGMAX, ymax, 0)) / 8);
+            if (selems < mx) selems = (unsigned)mx;
+            break;
+        }
+    }
+    if (b < 1) return doci_fail(DOCI_E_ARG, "no star split fits the LDS budget");
+    uint64_t emax = MAIN_EMAX;
+    for (unsigned k = 0; k <= N; ++k)
+        if (valid_sector(k, N, a, b) && dh_binom(N - k, b) > emax) emax = dh_binom(N - k, b);
+
+    /* sector offsets in Z and slab counts */
+    uint32_t *zoff = calloc(a + 2, sizeof(uint32_t));
+    uint64_t nslab = 0, nstar = 0, zacc = 0;
+    for (unsigned k = 0; k <= a; ++k) {
+        zoff[k] = (uint32_t)zacc;
+        if (valid_sector(k, N, a, b)) {
+            nslab += dh_binom(k, a);
+            zacc += dh_binom(k, a) * nslot(k, N - k, a) * dh_binom(N - k, b);
+        }
+    }
+    zoff[a + 1] = (uint32_t)zacc;
+    for (unsigned j = 0; j < N; ++j)
+        if (valid_sector(j + 1, N, a, b)) nstar += dh_binom(j, a);
+    int rc = DOCI_OK;
+    if (zacc >= (1ull << 32)) {
+        free(zoff);
+        return doci_fail(DOCI_E_DIM, "slot buffer exceeds 2^32 entries");
+    }
+    uint32_t *sstart = malloc(sizeof(uint32_t) * nslab), *szb = malloc(sizeof(uint32_t) * nslab);
+    sgrp_t *grp = malloc(sizeof(sgrp_t) * (nslab + 1));
+    uint64_t *samask = malloc(sizeof(uint64_t) * (nstar + 1));
+    uint32_t *sgw = malloc(sizeof(uint32_t) * 2 * (nstar + 1)), *gw = NULL;
+    dh_stars *s = NULL;
+    int idx[64];
+    if (!sstart || !szb || !grp || !samask || !sgw) {
+        rc = doci_fail(DOCI_E_NOMEM, "star tables");
+        goto out;
+    }
+    /* slabs and main-pass groups */
+    uint64_t is = 0, ng = 0;
+    for (unsigned k = 0; k <= a; ++k) {
+        if (!valid_sector(k, N, a, b)) continue;
+        const unsigned m = N - k;
+        const uint64_t S = dh_binom(m, b), ns = nslot(k, m, a);
+        for (unsigned e = 0; e < k; ++e) idx[e] = (int)e;
+        uint32_t gfirst = (uint32_t)is, gcount = 0;
+        uint64_t li = 0;
+        do {
+            uint64_t sa = 0;
+            for (unsigned e = 0; e < k; ++e) sa += dh_binom(N - e, L - 1 - (unsigned)idx[e]);
+            if (gcount == 0xffff || (gcount + 1) * S > emax) {
+                grp[ng].first = gfirst;
+                grp[ng].start = sstart[gfirst];
+                grp[ng].word = gcount | (m << 16);
+                ++ng;
+                gfirst = (uint32_t)is;
+                gcount = 0;
+            }
+            sstart[is] = (uint32_t)(n - S - sa);
+            szb[is] = (uint32_t)(zoff[k] + li * ns * S);
+            ++is;
+            ++li;
+            ++gcount;
+        } while (next_comb(idx, k, a));
+        grp[ng].first = gfirst;
+        grp[ng].start = sstart[gfirst];
+        grp[ng].word = gcount | (m << 16);
+        ++ng;
+    }
+    qsort(grp, ng, sizeof(sgrp_t), cmp_sgrp);
+    gw = malloc(sizeof(uint32_t) * 2 * ng);
+    if (!gw) {
+        rc = doci_fail(DOCI_E_NOMEM, "main groups");
+        goto out;
+    }
+    for (uint64_t g = 0; g < ng; ++g) {
+        gw[2 * g] = grp[g].first;
+        gw[2 * g + 1] = grp[g].word;
+    }
+    /* stars and star groups */
+    uint64_t ist = 0, nsg = 0;
+    for (unsigned j = 0; j < N; ++j) {
+        if (!valid_sector(j + 1, N, a, b)) continue;
+        const uint64_t stride = (uint64_t)(a - j) * dh_binom(N - j - 1, b) +
+                                (valid_sector(j, N, a, b) ? dh_binom(N - j, b) : 0);
+        for (unsigned e = 0; e < j; ++e) idx[e] = (int)e;
+        uint32_t gfirst = (uint32_t)ist, gcount = 0;
+        do {
+            uint64_t msk = 0;
+            for (unsigned e = 0; e < j; ++e) msk |= 1ull << idx[e];
+            if (gcount == STAR_SGMAX || (gcount + 1) * stride > selems) {
+                sgw[2 * nsg] = gfirst;
+                sgw[2 * nsg + 1] = gcount | (j << 16);
+                ++nsg;
+                gfirst = (uint32_t)ist;
+                gcount = 0;
+            }
+            samask[ist++] = msk;
+            ++gcount;
+        } while (next_comb(idx, j, a));
+        sgw[2 * nsg] = gfirst;
+        sgw[2 * nsg + 1] = gcount | (j << 16);
+        ++nsg;
+    }
+    s = calloc(1, sizeof *s);
+    if (!s) {
+        rc = doci_fail(DOCI_E_NOMEM, "star descriptor");
+        goto out;
+    }
+    {
+        const size_t b_am = sizeof(uint64_t) * nstar, b_ss = sizeof(uint32_t) * nslab,
+                     b_zb = sizeof(uint32_t) * nslab, b_g = sizeof(uint32_t) * 2 * ng,
+                     b_sg = sizeof(uint32_t) * 2 * nsg, b_zo = sizeof(uint32_t) * (a + 2);
+        hipError_t e = hipMalloc(&s->dmem, b_am + b_ss + b_zb + b_g + b_sg + b_zo);
+        char *d = s->dmem;
+        if (e == hipSuccess) e = hipMemcpy(d, samask, b_am, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d + b_am, sstart, b_ss, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d + b_am + b_ss, szb, b_zb, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d + b_am + b_ss + b_zb, gw, b_g, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(d + b_am + b_ss + b_zb + b_g, sgw, b_sg, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(d + b_am + b_ss + b_zb + b_g + b_sg, zoff, b_zo, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMalloc((void **)&s->d_bad, sizeof(uint32_t));
+        if (e == hipSuccess) e = hipMemset(s->d_bad, 0, sizeof(uint32_t));
+        if (e != hipSuccess) {
+            rc = doci_fail(DOCI_E_HIP, "star tables on device: %s", hipGetErrorString(e));
+            goto out;
+        }
+        s->d.a = a;
+        s->d.b = b;
+        s->d.ngroup = (uint32_t)ng;
+        s->d.emax = (uint32_t)emax;
+        s->d.nsgroup = (uint32_t)nsg;
+        s->d.sgmax = STAR_SGMAX;
+        s->d.selems = selems;
+        s->d.ymax = ymax;
+        s->d.star_amask = (const uint64_t *)d;
+        s->d.slab_start = (const uint32_t *)(d + b_am);
+        s->d.slab_zbase = (const uint32_t *)(d + b_am + b_ss);
+        s->d.group = (const uint32_t *)(d + b_am + b_ss + b_zb);
+        s->d.sgroup = (const uint32_t *)(d + b_am + b_ss + b_zb + b_g);
+        s->d.zoff = (const uint32_t *)(d + b_am + b_ss + b_zb + b_g + b_sg);
+        s->zelems = zacc;
+    }
+    if (dk_sigma_star_lds(&op->dt, &s->d, 0, 0) > STAR_LDS_BUDGET ||
+        dk_sigma_star_lds(&op->dt, &s->d, 0, 1) > STAR_LDS_BUDGET) {
+        rc = doci_fail(DOCI_E_ARG, "star kernels exceed the LDS budget");
+        goto out;
+    }
+    op->stars = s;
+    s = NULL;
+out:
+    if (s) {
+        if (s->dmem) hipFree(s->dmem);
+        if (s->d_bad) hipFree(s->d_bad);
+        free(s);
+    }
+    free(zoff);
+    free(sstart);
+    free(szb);
+    free(grp);
+    free(samask);
+    free(sgw);
+    free(gw);
+    return rc;
+}
+
+int dh_stars_sigma(doci_op *op, const void *in, void *out, int prec, double shift, int checked)
+{
+    dh_stars *s = op->stars;
+    if (!s->Z) { /* the slot buffer is allocated on first use and kept */
+        hipError_t e = hipMalloc(&s->Z, sizeof(double) * (s->zelems ? s->zelems : 1));
+        if (e != hipSuccess) {
+            s->Z = NULL;
+            return doci_fail(DOCI_E_NOMEM, "sigma slot buffer (%zu entries): %s", s->zelems,
+                             hipGetErrorString(e));
+        }
+    }
+    DKCHK(dk_sigma_star(&op->dt, &s->d, op->d_D, shift, in, out, s->Z, prec,
+                        checked ? s->d_bad : NULL, op->stream));
+    return DOCI_OK;
+}
+
+int64_t dh_stars_violations(doci_op *op)
+{
+    dh_stars *s = op->stars;
+    uint32_t v = 0;
+    if (!s) return doci_fail(DOCI_E_ARG, "no star tables");
+    HIPCHK(hipStreamSynchronize(op->stream));
+    HIPCHK(hipMemcpy(&v, s->d_bad, sizeof v, hipMemcpyDeviceToHost));
+    return (int64_t)v;
+}
+
+void dh_stars_info(const doci_op *op, unsigned *a, unsigned *b, unsigned *ngroup,
+                   unsigned *nsgroup, size_t *zelems)
+{
+    const dh_stars *s = op->stars;
+    *a = s ? s->d.a : 0;
+    *b = s ? s->d.b : 0;
+    *ngroup = s ? s->d.ngroup : 0;
+    *nsgroup = s ? s->d.nsgroup : 0;
+    *zelems = s ? s->zelems : 0;
+}

@@ -93,7 +93,7 @@ __global__ __launch_bounds__(kBlock) void k_sigma_slab(dk_tables t, dk_slabs sl,
     extern __shared__ double lds_raw[];
     // groups are dealt to XCDs in contiguous runs (workgroup i runs on XCD i % 8)
     const uint32_t per = (sl.ngroup + kXcd - 1) / kXcd;
-    const uint32_t g = (blockIdx.x % kXcd) * per + blockIdx.x / kXcd;
+    const uint32_t g = sl.xcd_runs ? (blockIdx.x % kXcd) * per + blockIdx.x / kXcd : blockIdx.x;
     if (g >= sl.ngroup) return; // uniform for the whole workgroup
     const int L = (int)t.L, N = (int)t.N, a = (int)sl.a, b = (int)sl.b;
     const uint32_t n = t.n;
@@ -190,11 +190,28 @@ __global__ __launch_bounds__(kBlock) void k_sigma_slab(dk_tables t, dk_slabs sl,
         {
             const uint32_t *as = p.aa_start + s * sl.aamax;
             const uint16_t *ao = p.aa_ot + s * sl.aamax;
-            for (int h = 0; h < nAA; ++h) {
+            // four independent partial sums keep four neighbour-slab loads in flight
+            double q0 = 0, q1 = 0, q2 = 0, q3 = 0;
+            int h = 0;
+            for (; h + 4 <= nAA; h += 4) {
+                const uint32_t g0 = as[h] + bi, g1 = as[h + 1] + bi, g2 = as[h + 2] + bi,
+                               g3 = as[h + 3] + bi;
+                const uint16_t o0 = ao[h], o1 = ao[h + 1], o2 = ao[h + 2], o3 = ao[h + 3];
+                const double v0 = ok(g0, n) ? (double)c[g0] : 0.0;
+                const double v1 = ok(g1, n) ? (double)c[g1] : 0.0;
+                const double v2 = ok(g2, n) ? (double)c[g2] : 0.0;
+                const double v3 = ok(g3, n) ? (double)c[g3] : 0.0;
+                q0 += K[(o0 & 0xff) * L + (o0 >> 8)] * v0;
+                q1 += K[(o1 & 0xff) * L + (o1 >> 8)] * v1;
+                q2 += K[(o2 & 0xff) * L + (o2 >> 8)] * v2;
+                q3 += K[(o3 & 0xff) * L + (o3 >> 8)] * v3;
+            }
+            for (; h < nAA; ++h) {
                 const uint16_t ot = ao[h];
                 const uint32_t gi = as[h] + bi;
-                if (ok(gi, n)) acc += K[(ot & 0xff) * L + (ot >> 8)] * (double)c[gi];
+                if (ok(gi, n)) q0 += K[(ot & 0xff) * L + (ot >> 8)] * (double)c[gi];
             }
+            acc += (q0 + q1) + (q2 + q3);
         }
         const uint32_t Sb = (uint32_t)S - 1u - (uint32_t)bi; // sum of beta's rank terms
         // AB: a pair leaves A for the empty B orbital tl
@@ -272,7 +289,7 @@ extern "C" int dk_sigma_slab_checked(const dk_tables *t, const dk_slabs *sl, con
 {
     if (sl->ngroup == 0) return (int)hipErrorInvalidValue;
     const uint32_t per = (sl->ngroup + kXcd - 1) / kXcd;
-    const dim3 grid(per * kXcd);
+    const dim3 grid(sl->xcd_runs ? per * kXcd : sl->ngroup);
     const size_t lds = prec ? lds_bytes<float>(*t, *sl) : lds_bytes<double>(*t, *sl);
     if (prec == 0 && !bad)
         hipLaunchKernelGGL((k_sigma_slab<double, false>), grid, dim3(kBlock), lds, s, *t, *sl, D,

@@ -24,27 +24,30 @@ def rel_err(a, b):
 
 
 def _checked_case(op, seed=3):
-    """Bounds-checked slab kernel first (no violation allowed), then the
-    unchecked slab and the lane kernel must agree with it."""
+    """Each blocked kernel runs bounds-checked first (no violation allowed);
+    then the unchecked kernels must reproduce the checked result exactly and
+    agree with the lane kernel to the FP64 bar."""
     c = da.gen_vector(op.n, seed)
-    op.set_kernel("slab-checked")
-    s_chk = op.sigma(c)
-    assert op.check_violations() == 0
-    op.set_kernel("slab")
-    s_slab = op.sigma(c)
     op.set_kernel("lane")
     s_lane = op.sigma(c)
-    op.set_kernel("slab")
-    assert np.array_equal(s_chk, s_slab)
-    assert rel_err(s_slab, s_lane) <= SIGMA_RTOL
-    return c, s_slab
+    for kern in ("star", "slab"):
+        op.set_kernel(kern + "-checked")
+        s_chk = op.sigma(c)
+        assert op.check_violations() == 0, kern
+        op.set_kernel(kern)
+        s = op.sigma(c)
+        assert np.array_equal(s_chk, s), kern
+        assert rel_err(s, s_lane) <= SIGMA_RTOL, kern
+    op.set_kernel("star")
+    return c, s_lane
 
 
 @pytest.mark.parametrize("name", SMALL + LARGE + ["syn", "L34"])
-def test_slab_kernel_bounds_then_parity(name, fcidump_dir, syn24, tmp_path_factory):
-    """Runs before every other sigma test: the slab kernel's index arithmetic
-    is proven in bounds on each shape used later (including L=34, N=10)."""
-    print(f"[case] slab checked {name}", flush=True)
+def test_blocked_kernels_bounds_then_parity(name, fcidump_dir, syn24, tmp_path_factory):
+    """Runs before every other sigma test: the star and slab kernels' index
+    arithmetic is proven in bounds on each shape used later (including L=34,
+    N=10), in FP64 and FP32."""
+    print(f"[case] blocked kernels checked {name}", flush=True)
     if name == "L34":
         op = da.Operator.synthetic(34, 10, 2)
     elif name == "syn":
@@ -54,12 +57,18 @@ def test_slab_kernel_bounds_then_parity(name, fcidump_dir, syn24, tmp_path_facto
     with op:
         a, b, g = op.slab_split()
         assert a + b == op.L and g > 0
+        sp = op.star_split()
+        assert sp["a"] + sp["b"] == op.L and sp["groups"] > 0
+        print(f"[case]   star split {sp}", flush=True)
         _checked_case(op)
-        # FP32 vectors through the checked kernel as well
-        op.set_kernel("slab-checked")
-        _sigma_fp32(op, da.gen_vector(op.n, 5))
-        assert op.check_violations() == 0
-        op.set_kernel("slab")
+        c5 = da.gen_vector(op.n, 5)
+        for kern in ("star", "slab"):
+            op.set_kernel(kern + "-checked")
+            s32 = _sigma_fp32(op, c5)
+            assert op.check_violations() == 0, kern
+            op.set_kernel(kern)
+            assert np.array_equal(_sigma_fp32(op, c5), s32)
+        op.set_kernel("star")
 
 
 def _sigma_fp32(op, c):
@@ -278,7 +287,7 @@ def test_sigma_fp32_storage(fcidump_dir):
     with da.Operator.from_fcidump(path) as op:
         c = da.gen_vector(op.n, 3)
         ref = op.sigma(c)
-        for kern in ("slab", "lane"):
+        for kern in ("star", "slab", "lane"):
             op.set_kernel(kern)
             out = _sigma_fp32(op, c)
             assert rel_err(out.astype(np.float64), ref) < 3e-7
