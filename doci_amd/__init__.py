@@ -249,7 +249,8 @@ class Operator:
     KERNELS = {"star": 0, "lane": 1, "star-checked": 2, "slab": 3, "slab-checked": 4}
 
     def set_kernel(self, name: str) -> None:
-        """Sigma kernel: "star" (default), "lane", "slab", or a "-checked" variant."""
+        """Sigma kernel: "star" (default), "slab" (bit-reproducible), "lane", or a
+        "-checked" variant."""
         _check(lib().doci_op_set_kernel(self._h, self.KERNELS[name]), "doci_op_set_kernel")
 
     def slab_split(self):

@@ -92,6 +92,18 @@ typedef struct {
     const uint64_t *star_amask; /* stars (order: j, then lexicographic rho) */
     const uint32_t *sgroup;     /* star groups: first star, nstar | j << 16 */
     const uint32_t *zoff;       /* [a + 2]: first Z index of each k-sector */
+    uint32_t jgroup[DK_MAX_L + 1]; /* star groups of j: [jgroup[j], jgroup[j+1]), j < N */
+    /* Tables over the masks beta of m pairs in B (b <= 16), r = lexB(beta):
+     * bmask[bmask_off[m] + r] = beta;
+     * hop[hop_off[m] + h * C(b,m) + r] = J | (o * b + t) << 16 for the h-th
+     *   hop beta - o + t (o in beta ascending, then t not in beta ascending),
+     *   J = its lexB rank;
+     * ins[ins_off[m] + t * C(b,m) + r] = lexB rank of beta + t among the
+     *   (m+1)-pair masks, 0xffff when t is in beta (m < b). */
+    const uint16_t *bmask;
+    const uint32_t *hop;
+    const uint16_t *ins;
+    uint32_t bmask_off[17], hop_off[17], ins_off[17];
 } dk_stars;
 
 /* LDS bytes of the star pass and of the main pass (prec as for dk_sigma). */

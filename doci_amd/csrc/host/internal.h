@@ -106,8 +106,9 @@ void dh_slabs_info(const doci_op *op, unsigned *a, unsigned *b, unsigned *ngroup
 int64_t dh_slabs_violations(doci_op *op);
 
 /* stars.c */
-int dh_stars_build(doci_op *op);
+int dh_stars_build(doci_op *op, unsigned lds_budget, unsigned sgmax, unsigned main_emax);
 void dh_stars_free(doci_op *op);
+void dh_stars_limits(const doci_op *op, unsigned *lds_budget, unsigned *sgmax, unsigned *main_emax);
 int dh_stars_sigma(doci_op *op, const void *in, void *out, int prec, double shift, int checked);
 int64_t dh_stars_violations(doci_op *op);
 void dh_stars_info(const doci_op *op, unsigned *a, unsigned *b, unsigned *ngroup,

@@ -161,8 +161,9 @@ int doci_op_from_integrals(unsigned L, unsigned N, double core, const double *on
     TRY(hipStreamSynchronize(op->stream));
 #undef TRY
     rc = dh_slabs_build(op, 0, 0, 0);
-    if (rc == DOCI_OK) rc = dh_stars_build(op);
+    if (rc == DOCI_OK) rc = dh_stars_build(op, 0, 0, 0);
     if (rc != DOCI_OK) goto fail;
+    op->sigma_kind = 0; /* star kernel: the fastest measured at L=34, N=10 */
     *out = op;
     return DOCI_OK;
 fail:

@@ -48,11 +48,12 @@ int dh_slabs_build(doci_op *op, unsigned emax_req, unsigned gmax, int xcd_runs)
     const uint64_t n = (uint64_t)op->n;
     unsigned emax = emax_req ? emax_req : SLAB_EMAX;
     if (!gmax) gmax = SLAB_GMAX;
-    /* LDS: K table, binomials, FP64 staging, neighbour tables */
+    /* LDS: K table, binomials, FP64 staging of emax determinants and the
+     * per-slab neighbour tables of gmax slabs; b is the largest split whose
+     * biggest slab fits emax, and gmax shrinks until the tables fit too. */
     const size_t fixed = 8 * (size_t)L * L + 4 * (size_t)(L + 1) * (N + 2) + 64;
-    while (emax > 256 && fixed + 8 * (size_t)emax + gmax * (size_t)(L * L / 4 + 2 * L) * 6 >
-                             SLAB_LDS_BUDGET)
-        emax /= 2;
+    if (fixed + 8 * (size_t)emax > SLAB_LDS_BUDGET)
+        emax = (unsigned)((SLAB_LDS_BUDGET - fixed) / 8 / 2);
     unsigned b = L;
     for (; b >= 1; --b) {
         const unsigned a = L - b;
@@ -74,6 +75,10 @@ int dh_slabs_build(doci_op *op, unsigned emax_req, unsigned gmax, int xcd_runs)
         if (k > abmax) abmax = k;
         if (a - k > bamax) bamax = a - k;
     }
+    while (gmax > 1 && fixed + 8 * (size_t)emax +
+                               (size_t)gmax * (6 * (size_t)aamax + 5 * (size_t)(abmax + bamax) + 48) >
+                           SLAB_LDS_BUDGET)
+        --gmax;
     if (aamax == 0) aamax = 1;
     uint32_t *start = malloc(sizeof(uint32_t) * nslab);
     uint64_t *amask = malloc(sizeof(uint64_t) * nslab);
@@ -203,6 +208,17 @@ int dh_slabs_sigma(doci_op *op, const void *in, void *out, int prec, double shif
 int doci_op_tune(doci_op *op, const char *key, int value)
 {
     dh_slabs *s = op->slabs;
+    if (!strcmp(key, "star_lds") || !strcmp(key, "star_sgmax") || !strcmp(key, "star_emax")) {
+        /* rebuild the star tables with one limit changed (0 = default) */
+        static const char *keys[3] = {"star_lds", "star_sgmax", "star_emax"};
+        unsigned cur[3];
+        dh_stars_limits(op, &cur[0], &cur[1], &cur[2]);
+        for (int i = 0; i < 3; ++i)
+            if (!strcmp(key, keys[i])) cur[i] = (unsigned)value;
+        if (value < 0 || (cur[1] && cur[1] > 0xffff)) return doci_fail(DOCI_E_ARG, "bad %s", key);
+        dh_stars_free(op);
+        return dh_stars_build(op, cur[0], cur[1], cur[2]);
+    }
     if (!s) return doci_fail(DOCI_E_ARG, "no slab tables");
     if (!strcmp(key, "xcd_runs")) {
         s->d.xcd_runs = value != 0;

@@ -1,29 +1,32 @@
 /* stars.c -- tables of the two-pass (star / main) sigma.
  *
- * b is the largest number of top orbitals (<= 16) for which the biggest star
- * (a - j member slabs of C(b, N-j-1) plus the centre slab of C(b, N-j)) fits
- * the star pass's LDS with FP64 staging; a larger b means fewer A pairs per
- * determinant and so fewer Z slots to write and read.  Slabs are stored by
- * k (pairs in A), then in lexicographic order of their A-part, so a slab's
- * Z offset follows from its index in the sector.  Main-pass groups pack
- * whole slabs of one sector and are ordered by rank; star groups pack whole
- * stars of one j. */
+ * b is the largest number of top orbitals (<= 16) for which the biggest
+ * centre slab C(b, N-j) fits the star pass's LDS (FP64 value + FP64
+ * accumulator per element) and the biggest slab fits the main pass's; a
+ * larger b means fewer A pairs per determinant and so fewer Z slots to write
+ * and read.  Slabs are stored by k (pairs in A), then in lexicographic order
+ * of their A-part, so a slab's Z offset follows from its index in the sector.
+ * Main-pass groups pack whole slabs of one sector and are ordered by rank;
+ * star groups pack whole stars of one j. */
 #include <stdlib.h>
 #include <string.h>
 
 #include "internal.h"
 
-#define STAR_LDS_BUDGET (64 * 1024)
+#define STAR_LDS_DEFAULT (52 * 1024) /* three star-pass workgroups per CU */
+#define MAIN_LDS_MAX (64 * 1024)
 #define STAR_BMAX 16
-#define STAR_SGMAX 64
-#define MAIN_EMAX 4096
+#define STAR_SGMAX_DEFAULT 64
+#define MAIN_EMAX_DEFAULT 1024  /* raised to the biggest slab */
 
 typedef struct {
     dk_stars d;
     void *dmem;
     uint32_t *d_bad;
+    void *dtab;     /* B-mask, hop and insert tables */
     void *Z;        /* slot buffer, FP64-sized so both precisions fit */
     size_t zelems;
+    unsigned lim[3]; /* requested LDS budget, stars per group, main emax (0 = default) */
 } dh_stars;
 
 typedef struct {
@@ -46,8 +49,8 @@ static unsigned nslot(unsigned k, unsigned m, unsigned a) { return k + ((m >= 1 
 static size_t star_lds_bytes(unsigned L, unsigned N, unsigned sgmax, unsigned ymax, unsigned selems)
 {
 #define A8(x) (((size_t)(x) + 7) & ~(size_t)7)
-    return A8(8 * (size_t)L * L) + A8(4 * (size_t)(L + 1) * (N + 2)) + A8(8 * (size_t)selems) +
-           2 * A8(4 * (size_t)sgmax * ymax) + 2 * A8(4 * (size_t)sgmax) + 2 * A8((size_t)sgmax * ymax);
+    return A8(8 * (size_t)L * L) + A8(4 * (size_t)(L + 1) * (N + 2)) + 2 * A8(8 * (size_t)selems) +
+           2 * A8(4 * (size_t)sgmax * ymax) + 2 * A8(4 * (size_t)sgmax) + A8((size_t)sgmax * ymax);
 #undef A8
 }
 
@@ -56,6 +59,7 @@ void dh_stars_free(doci_op *op)
     dh_stars *s = op->stars;
     if (!s) return;
     if (s->dmem) hipFree(s->dmem);
+    if (s->dtab) hipFree(s->dtab);
     if (s->d_bad) hipFree(s->d_bad);
     if (s->Z) hipFree(s->Z);
     free(s);
@@ -73,19 +77,111 @@ static int next_comb(int *idx, unsigned k, unsigned a)
     return 1;
 }
 
-int dh_stars_build(doci_op *op)
+/* lexicographic rank of the m-subset idx[0..m) of {0..b-1} */
+static uint32_t lex_rank(const int *idx, unsigned m, unsigned b)
 {
+    uint64_t s = 0;
+    for (unsigned e = 0; e < m; ++e) s += dh_binom(m - e, b - 1 - (unsigned)idx[e]);
+    return (uint32_t)(dh_binom(m, b) - 1 - s);
+}
+
+/* The B-mask, hop and insert tables of dk_stars, uploaded to s->dtab. */
+static int build_b_tables(dh_stars *s, unsigned b)
+{
+    size_t nmask = 0, nhop = 0, nins = 0;
+    for (unsigned m = 0; m <= b; ++m) {
+        const size_t S = dh_binom(m, b);
+        s->d.bmask_off[m] = (uint32_t)nmask;
+        s->d.hop_off[m] = (uint32_t)nhop;
+        s->d.ins_off[m] = (uint32_t)nins;
+        nmask += S;
+        nhop += S * m * (b - m);
+        if (m < b) nins += S * b;
+    }
+    const size_t bytes_m = sizeof(uint16_t) * nmask, bytes_h = sizeof(uint32_t) * nhop,
+                 bytes_i = sizeof(uint16_t) * nins;
+    const size_t o_h = (bytes_m + 255) & ~(size_t)255, o_i = (o_h + bytes_h + 255) & ~(size_t)255;
+    char *h = malloc(o_i + bytes_i + 1);
+    if (!h) return doci_fail(DOCI_E_NOMEM, "B tables");
+    uint16_t *bm = (uint16_t *)h, *ins = (uint16_t *)(h + o_i);
+    uint32_t *hop = (uint32_t *)(h + o_h);
+    int idx[17], jdx[17];
+    for (unsigned m = 0; m <= b; ++m) {
+        const size_t S = dh_binom(m, b);
+        for (unsigned e = 0; e < m; ++e) idx[e] = (int)e;
+        size_t r = 0;
+        do {
+            unsigned mask = 0;
+            for (unsigned e = 0; e < m; ++e) mask |= 1u << idx[e];
+            bm[s->d.bmask_off[m] + r] = (uint16_t)mask;
+            size_t hh = 0;
+            for (unsigned eo = 0; eo < m; ++eo) {
+                const int o = idx[eo];
+                for (unsigned t = 0; t < b; ++t) {
+                    if ((mask >> t) & 1) continue;
+                    /* beta - o + t, sorted */
+                    unsigned q = 0;
+                    for (unsigned e = 0; e < m; ++e)
+                        if (idx[e] != o) jdx[q++] = idx[e];
+                    unsigned pos = q;
+                    while (pos > 0 && jdx[pos - 1] > (int)t) {
+                        jdx[pos] = jdx[pos - 1];
+                        --pos;
+                    }
+                    jdx[pos] = (int)t;
+                    const uint32_t J = lex_rank(jdx, m, b);
+                    hop[s->d.hop_off[m] + hh * S + r] = J | (uint32_t)(o * b + t) << 16;
+                    ++hh;
+                }
+            }
+            if (m < b)
+                for (unsigned t = 0; t < b; ++t) {
+                    uint16_t v = 0xffff;
+                    if (!((mask >> t) & 1)) {
+                        unsigned q = 0, done = 0;
+                        for (unsigned e = 0; e < m; ++e) {
+                            if (!done && idx[e] > (int)t) {
+                                jdx[q++] = (int)t;
+                                done = 1;
+                            }
+                            jdx[q++] = idx[e];
+                        }
+                        if (!done) jdx[q++] = (int)t;
+                        v = (uint16_t)lex_rank(jdx, m + 1, b);
+                    }
+                    ins[s->d.ins_off[m] + (size_t)t * S + r] = v;
+                }
+            ++r;
+        } while (m > 0 && next_comb(idx, m, b));
+    }
+    hipError_t e = hipMalloc(&s->dtab, o_i + bytes_i + 1);
+    if (e == hipSuccess) e = hipMemcpy(s->dtab, h, o_i + bytes_i, hipMemcpyHostToDevice);
+    free(h);
+    if (e != hipSuccess) {
+        s->dtab = NULL;
+        return doci_fail(DOCI_E_HIP, "B tables on device: %s", hipGetErrorString(e));
+    }
+    s->d.bmask = (const uint16_t *)s->dtab;
+    s->d.hop = (const uint32_t *)((char *)s->dtab + o_h);
+    s->d.ins = (const uint16_t *)((char *)s->dtab + o_i);
+    return DOCI_OK;
+}
+
+int dh_stars_build(doci_op *op, unsigned lds_budget, unsigned sgmax, unsigned main_emax)
+{
+    const size_t STAR_LDS_BUDGET = lds_budget ? lds_budget : STAR_LDS_DEFAULT;
+    const unsigned STAR_SGMAX = sgmax ? sgmax : STAR_SGMAX_DEFAULT;
+    const unsigned MAIN_EMAX = main_emax ? main_emax : MAIN_EMAX_DEFAULT;
     const unsigned L = op->L, N = op->N;
     const uint64_t n = (uint64_t)op->n;
     unsigned b = L < STAR_BMAX ? L : STAR_BMAX, a = 0, ymax = 0, selems = 0;
     for (; b >= 1; --b) {
         a = L - b;
-        uint64_t mx = 0;
+        uint64_t mx = 0; /* biggest centre slab */
         ymax = 1;
         for (unsigned j = 0; j < N; ++j) {
             if (!valid_sector(j + 1, N, a, b)) continue; /* members' sector */
-            const uint64_t st = (uint64_t)(a - j) * dh_binom(N - j - 1, b) +
-                                (valid_sector(j, N, a, b) ? dh_binom(N - j, b) : 0);
+            const uint64_t st = valid_sector(j, N, a, b) ? dh_binom(N - j, b) : 0;
             if (st > mx) mx = st;
             if (a - j > ymax) ymax = a - j;
         }
@@ -95,10 +191,10 @@ int dh_stars_build(doci_op *op)
         if (maxS < MAIN_EMAX) maxS = MAIN_EMAX;
         if (ymax > 64) continue;
         if (star_lds_bytes(L, N, STAR_SGMAX, ymax, (unsigned)mx) <= STAR_LDS_BUDGET &&
-            8 * (size_t)L * L + 4 * (size_t)(L + 1) * (N + 2) + 8 * (size_t)maxS + 64 <=
-                STAR_LDS_BUDGET) {
-            /* the rest of the budget stages as many stars as fit */
-            selems = (unsigned)((STAR_LDS_BUDGET - star_lds_bytes(L, N, STAR_SGMAX, ymax, 0)) / 8);
+            8 * (size_t)b * b + 4 * (size_t)(b + 1) * (N + 2) + 8 * (size_t)maxS + 64 <=
+                MAIN_LDS_MAX) {
+            /* the rest of the budget stages as many centres as fit */
+            selems = (unsigned)((STAR_LDS_BUDGET - star_lds_bytes(L, N, STAR_SGMAX, ymax, 0)) / 16);
             if (selems < mx) selems = (unsigned)mx;
             break;
         }
@@ -179,10 +275,11 @@ int dh_stars_build(doci_op *op)
     }
     /* stars and star groups */
     uint64_t ist = 0, nsg = 0;
+    uint32_t jgroup[DK_MAX_L + 1];
     for (unsigned j = 0; j < N; ++j) {
+        jgroup[j] = (uint32_t)nsg;
         if (!valid_sector(j + 1, N, a, b)) continue;
-        const uint64_t stride = (uint64_t)(a - j) * dh_binom(N - j - 1, b) +
-                                (valid_sector(j, N, a, b) ? dh_binom(N - j, b) : 0);
+        const uint64_t stride = valid_sector(j, N, a, b) ? dh_binom(N - j, b) : 0;
         for (unsigned e = 0; e < j; ++e) idx[e] = (int)e;
         uint32_t gfirst = (uint32_t)ist, gcount = 0;
         do {
@@ -233,6 +330,8 @@ int dh_stars_build(doci_op *op)
         s->d.emax = (uint32_t)emax;
         s->d.nsgroup = (uint32_t)nsg;
         s->d.sgmax = STAR_SGMAX;
+        jgroup[N] = (uint32_t)nsg;
+        memcpy(s->d.jgroup, jgroup, sizeof(uint32_t) * (N + 1));
         s->d.selems = selems;
         s->d.ymax = ymax;
         s->d.star_amask = (const uint64_t *)d;
@@ -242,9 +341,14 @@ int dh_stars_build(doci_op *op)
         s->d.sgroup = (const uint32_t *)(d + b_am + b_ss + b_zb + b_g);
         s->d.zoff = (const uint32_t *)(d + b_am + b_ss + b_zb + b_g + b_sg);
         s->zelems = zacc;
+        s->lim[0] = lds_budget;
+        s->lim[1] = sgmax;
+        s->lim[2] = main_emax;
     }
+    rc = build_b_tables(s, b);
+    if (rc != DOCI_OK) goto out;
     if (dk_sigma_star_lds(&op->dt, &s->d, 0, 0) > STAR_LDS_BUDGET ||
-        dk_sigma_star_lds(&op->dt, &s->d, 0, 1) > STAR_LDS_BUDGET) {
+        dk_sigma_star_lds(&op->dt, &s->d, 0, 1) > MAIN_LDS_MAX) {
         rc = doci_fail(DOCI_E_ARG, "star kernels exceed the LDS budget");
         goto out;
     }
@@ -253,6 +357,7 @@ int dh_stars_build(doci_op *op)
 out:
     if (s) {
         if (s->dmem) hipFree(s->dmem);
+        if (s->dtab) hipFree(s->dtab);
         if (s->d_bad) hipFree(s->d_bad);
         free(s);
     }
@@ -290,6 +395,14 @@ int64_t dh_stars_violations(doci_op *op)
     HIPCHK(hipStreamSynchronize(op->stream));
     HIPCHK(hipMemcpy(&v, s->d_bad, sizeof v, hipMemcpyDeviceToHost));
     return (int64_t)v;
+}
+
+void dh_stars_limits(const doci_op *op, unsigned *lds_budget, unsigned *sgmax, unsigned *main_emax)
+{
+    const dh_stars *s = op->stars;
+    *lds_budget = s ? s->lim[0] : 0;
+    *sgmax = s ? s->lim[1] : 0;
+    *main_emax = s ? s->lim[2] : 0;
 }
 
 void dh_stars_info(const doci_op *op, unsigned *a, unsigned *b, unsigned *ngroup,

@@ -6,15 +6,15 @@
 // starting at n - C(b,m) - sum_e C(L-1-alpha_e, N-e), in the order lexB(beta).
 //
 // Star pass.  The star of rho (j pairs in A) is its centre slab rho (m = N-j
-// pairs in B, absent when N - j > b) and the a - j member slabs rho + y.  One
-// workgroup stages whole stars in LDS.  For member rho + x it sums every hop
+// pairs in B, absent when N - j > b) and the a - j member slabs rho + y.  A
+// lane per (star, member offset beta) loads the members' values at beta into
+// registers (coalesced across lanes).  For member rho + x it sums every hop
 // that empties x:  x -> y in A  (sum_y K_xy X_y[beta]: a small dense product
-// over the staged members, same offset) and  x -> t in B  (K_xt times the
-// centre at lexB(beta + t));  for the centre it sums every hop from B into A
-// (K_oy times member y at lexB(beta - o)).  Each sum is one slot of Z.
-// Every slab is staged k + 1 times per sigma (k stars as member, one as
-// centre) with full-line coalesced loads, instead of being re-read once per
-// neighbouring determinant.
+// in registers) and  x -> t in B  (K_xt times the centre, staged in LDS, at
+// lexB(beta + t));  at the same centre offsets it accumulates in LDS the
+// centre's hops from B into A.  Each sum is one slot of Z.  Every slab is
+// read k + 1 times per sigma (k stars as member, one as centre) with
+// full-line coalesced loads, instead of once per neighbouring determinant.
 //
 // Main pass.  sigma = (D - shift) c + hops inside the slab (own slab staged in
 // LDS) + the slab's Z slots (coalesced streams).
@@ -63,11 +63,14 @@ __device__ __forceinline__ int nslot_of(int k, int m, int a) { return k + ((m >=
 struct StarLds {
     double *K;
     uint32_t *bt;
-    void *stage;
+    double *acc;   // centre accumulators (hops from B into A), per star
+    void *ctr;     // staged centre slabs, per star
     uint32_t *mem_start, *mem_zb, *ctr_start, *ctr_zb;
-    uint8_t *ycol, *mpos;
+    uint8_t *ycol;
 };
 
+// Star pass LDS: K, binomials, per star the centre slab (values + FP64
+// accumulator, together `selems` elements per group) and member tables.
 template <typename T>
 __host__ __device__ size_t star_lds(uint32_t L, uint32_t N, const dk_stars &st, StarLds *p, char *base)
 {
@@ -79,40 +82,54 @@ __host__ __device__ size_t star_lds(uint32_t L, uint32_t N, const dk_stars &st, 
     };
     char *K = take(sizeof(double) * L * L);
     char *bt = take(sizeof(uint32_t) * (L + 1) * (N + 2));
-    char *stg = take(sizeof(T) * st.selems);
+    char *acc = take(sizeof(double) * st.selems);
+    char *ctr = take(sizeof(T) * st.selems);
     char *ms = take(sizeof(uint32_t) * st.sgmax * st.ymax);
     char *mz = take(sizeof(uint32_t) * st.sgmax * st.ymax);
     char *cs = take(sizeof(uint32_t) * st.sgmax);
     char *cz = take(sizeof(uint32_t) * st.sgmax);
     char *yc = take(st.sgmax * st.ymax);
-    char *mp = take(st.sgmax * st.ymax);
     if (p) {
         p->K = (double *)K;
         p->bt = (uint32_t *)bt;
-        p->stage = stg;
+        p->acc = (double *)acc;
+        p->ctr = ctr;
         p->mem_start = (uint32_t *)ms;
         p->mem_zb = (uint32_t *)mz;
         p->ctr_start = (uint32_t *)cs;
         p->ctr_zb = (uint32_t *)cz;
         p->ycol = (uint8_t *)yc;
-        p->mpos = (uint8_t *)mp;
     }
     return off;
 }
 
+// Main pass LDS: K restricted to B x B, binomials C(x, j) for x <= b, the
+// staged slabs.  Small, so that many workgroups share a CU.
 template <typename T>
 __host__ __device__ size_t main_lds(uint32_t L, uint32_t N, const dk_stars &st)
 {
-    return a8(sizeof(double) * L * L) + a8(sizeof(uint32_t) * (L + 1) * (N + 2)) +
+    return a8(sizeof(double) * st.b * st.b) + a8(sizeof(uint32_t) * (st.b + 1) * (N + 2)) +
            a8(sizeof(T) * st.emax);
 }
 
+// One lane per (star, member offset beta).  The lane loads the member slabs'
+// values at beta (coalesced across lanes) into registers and produces, for
+// every member rho + x, the sum of the hops that empty x:
+//   sum_y K_xy X_y[beta]            (A -> A, a dense |Y| x |Y| product)
+// + sum_t K_xt ctr[lexB(beta + t)]  (A -> B, centre staged in LDS)
+// and adds to the centre accumulator at lexB(beta + t) the hops of the
+// centre from B orbital t into A:  sum_y K_ty X_y[beta].
+// The products run over YMAX >= |Y| members and BMAX >= b orbitals with zero
+// padding, so that they are straight-line code (no per-term branches) whose
+// LDS reads the compiler can batch.  Groups [g0, g0 + gridDim.x) all have
+// a - j <= YMAX.
 template <typename T, bool kChecked, int YMAX, int BMAX>
-__global__ __launch_bounds__(kBlock) void k_star(dk_tables t, dk_stars st, const T *__restrict__ c,
-                                                 T *__restrict__ Z, uint32_t *bad)
+__global__ __launch_bounds__(kBlock, (YMAX <= 24 ? 3 : 2)) void k_star(dk_tables t, dk_stars st, uint32_t g0,
+                                                    const T *__restrict__ c, T *__restrict__ Z,
+                                                    uint32_t *bad)
 {
     extern __shared__ double lds_raw[];
-    const uint32_t sg = blockIdx.x;
+    const uint32_t sg = g0 + blockIdx.x;
     if (sg >= st.nsgroup) return;
     StarLds p;
     star_lds<T>(t.L, t.N, st, &p, (char *)lds_raw);
@@ -134,13 +151,17 @@ __global__ __launch_bounds__(kBlock) void k_star(dk_tables t, dk_stars st, const
     const bool has_ctr = mc <= b;
     const int mm = mc - 1;              // B pairs of the members
     const int Sm = (int)C(b, mm), Sc = has_ctr ? (int)C(b, mc) : 0;
-    const int ny = a - j, stride = ny * Sm + Sc;
+    const int ny = a - j;
+    if (kChecked && (ny > YMAX || b > BMAX)) {
+        if (threadIdx.x == 0) atomicAdd(bad, 1u);
+        return;
+    }
     const int km = j + 1;
     const uint32_t zm_sector = st.zoff[km], zc_sector = st.zoff[j];
     const int nsm = nslot_of(km, mm, a), nsc = nslot_of(j, mc, a);
     const uint64_t amask_all = a < 64 ? ((1ull << a) - 1) : ~0ull;
 
-    // member / centre tables
+    // member tables: first rank, Z position of this star's slot, orbital y
     for (int task = threadIdx.x; task < nst * ny; task += kBlock) {
         const int s = task / ny, yi = task - s * ny;
         const uint64_t rho = st.star_amask[first + s];
@@ -148,10 +169,11 @@ __global__ __launch_bounds__(kBlock) void k_star(dk_tables t, dk_stars st, const
         for (int q = 0; q < yi; ++q) z &= z - 1;
         const int y = __ffsll((unsigned long long)z) - 1;
         const uint64_t mem = rho | (1ull << y);
+        const int pos = __popcll(rho & ((1ull << y) - 1));
         p.mem_start[s * ny + yi] = slab_first(mem, mm, L, N, b, n, C);
-        p.mem_zb[s * ny + yi] = zm_sector + lex_a(mem, km, a, C) * (uint32_t)(nsm * Sm);
+        p.mem_zb[s * ny + yi] =
+            zm_sector + lex_a(mem, km, a, C) * (uint32_t)(nsm * Sm) + (uint32_t)(pos * Sm);
         p.ycol[s * ny + yi] = (uint8_t)y;
-        p.mpos[s * ny + yi] = (uint8_t)__popcll(rho & ((1ull << y) - 1));
     }
     if (has_ctr)
         for (int s = threadIdx.x; s < nst; s += kBlock) {
@@ -160,92 +182,87 @@ __global__ __launch_bounds__(kBlock) void k_star(dk_tables t, dk_stars st, const
             p.ctr_zb[s] = zc_sector + lex_a(rho, j, a, C) * (uint32_t)(nsc * Sc) + (uint32_t)(j * Sc);
         }
     __syncthreads();
-    T *stage = (T *)p.stage;
-    for (int e = threadIdx.x; e < nst * stride; e += kBlock) {
-        const int s = e / stride, r = e - s * stride;
-        uint32_t gi;
-        if (r < ny * Sm) {
-            const int yi = r / Sm;
-            gi = p.mem_start[s * ny + yi] + (uint32_t)(r - yi * Sm);
-        } else {
-            gi = p.ctr_start[s] + (uint32_t)(r - ny * Sm);
+    T *ctrv = (T *)p.ctr;
+    if (has_ctr)
+        for (int e = threadIdx.x; e < nst * Sc; e += kBlock) {
+            const int s = e / Sc;
+            const uint32_t gi = p.ctr_start[s] + (uint32_t)(e - s * Sc);
+            ctrv[e] = ok(gi, n) ? c[gi] : (T)0;
+            p.acc[e] = 0.0;
         }
-        stage[e] = ok(gi, n) ? c[gi] : (T)0;
-    }
     __syncthreads();
 
     const double *K = p.K;
-    // members: every hop that empties x, for each member rho + x
     for (int it = threadIdx.x; it < nst * Sm; it += kBlock) {
         const int s = it / Sm, bb = it - s * Sm;
-        const T *X = stage + s * stride;
-        const T *ctr = X + ny * Sm;
         const uint8_t *yc = p.ycol + s * ny;
+        const uint32_t *ms = p.mem_start + s * ny;
+        // members beyond ny read member 0 again and count as zero
         double xv[YMAX];
+        int yr[YMAX];
 #pragma unroll
-        for (int yi = 0; yi < YMAX; ++yi) xv[yi] = yi < ny ? (double)X[yi * Sm + bb] : 0.0;
-        const uint64_t bm = dk::unrank((uint32_t)bb, b, mm, C);
+        for (int yi = 0; yi < YMAX; ++yi) {
+            const int q = yi < ny ? yi : 0;
+            yr[yi] = yc[q];
+            const uint32_t gi = ms[q] + (uint32_t)bb;
+            const double v = ok(gi, n) ? (double)c[gi] : 0.0;
+            xv[yi] = yi < ny ? v : 0.0;
+        }
+        // centre values at lexB(beta + t): zero where t is in beta, t >= b or
+        // there is no centre; orbital rows past b read row b - 1
+        double cv[BMAX];
         uint32_t offs[BMAX];
-        if (has_ctr) { // lexB of beta + t in the centre (mm + 1 pairs)
-            const uint32_t Sb = (uint32_t)Sm - 1u - (uint32_t)bb;
-            uint32_t P1 = 0, P2 = 0;
-            int jj = 0;
+        bool live[BMAX];
+        {
+            // lexB(beta + t) from the insert table (coalesced across lanes)
+            const uint16_t *ip = st.ins + st.ins_off[has_ctr ? mm : 0] + bb;
 #pragma unroll
             for (int tl = 0; tl < BMAX; ++tl) {
-                offs[tl] = 0;
-                if (tl < b) {
-                    if ((bm >> tl) & 1) {
-                        P2 += C(b - 1 - tl, mm + 1 - jj);
-                        P1 += C(b - 1 - tl, mm - jj);
-                        ++jj;
-                    } else {
-                        offs[tl] = (uint32_t)Sc - 1u - P2 - C(b - 1 - tl, mm + 1 - jj) - (Sb - P1);
-                    }
+                const uint32_t off = (has_ctr && tl < b) ? ip[(size_t)tl * Sm] : 0xffffu;
+                live[tl] = off != 0xffffu && ok(off, (uint32_t)Sc);
+                offs[tl] = live[tl] ? off : 0;
+                cv[tl] = live[tl] ? (double)ctrv[s * Sc + offs[tl]] : 0.0;
+            }
+        }
+        // the centre's hops from B orbital tl into A, landing on the members at beta
+        if (has_ctr) {
+            double *acc = p.acc + s * Sc;
+#pragma unroll
+            for (int tl = 0; tl < BMAX; ++tl) {
+                const double *Kt = K + (a + (tl < b ? tl : b - 1)) * L;
+                double u0 = 0, u1 = 0;
+#pragma unroll
+                for (int yi = 0; yi < YMAX; yi += 2) {
+                    u0 += Kt[yr[yi]] * xv[yi];
+                    if (yi + 1 < YMAX) u1 += Kt[yr[yi + 1]] * xv[yi + 1];
                 }
+                if (live[tl])
+                    __hip_atomic_fetch_add(&acc[offs[tl]], u0 + u1, __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
             }
         }
         for (int xi = 0; xi < ny; ++xi) {
             const double *Kx = K + yc[xi] * L;
-            double acc = 0;
+            double a0 = 0, a1 = 0, a2 = 0;
 #pragma unroll
-            for (int yi = 0; yi < YMAX; ++yi)
-                if (yi < ny) acc += Kx[yc[yi]] * xv[yi];
-            if (has_ctr) {
-#pragma unroll
-                for (int tl = 0; tl < BMAX; ++tl)
-                    if (tl < b && !((bm >> tl) & 1) && ok(offs[tl], (uint32_t)Sc))
-                        acc += Kx[a + tl] * (double)ctr[offs[tl]];
+            for (int yi = 0; yi < YMAX; yi += 2) {
+                a0 += Kx[yr[yi]] * xv[yi];
+                if (yi + 1 < YMAX) a1 += Kx[yr[yi + 1]] * xv[yi + 1];
             }
-            const uint32_t zi = p.mem_zb[s * ny + xi] + (uint32_t)(p.mpos[s * ny + xi] * Sm + bb);
-            if (ok(zi, ztot)) Z[zi] = (T)acc;
+#pragma unroll
+            for (int tl = 0; tl < BMAX; ++tl) a2 += Kx[a + (tl < b ? tl : b - 1)] * cv[tl];
+            const uint32_t zi = p.mem_zb[s * ny + xi] + (uint32_t)bb;
+            if (ok(zi, ztot)) Z[zi] = (T)((a0 + a1) + a2);
         }
     }
-    // centre: every hop from a B orbital o of beta into an empty A orbital y
-    if (has_ctr)
-        for (int it = threadIdx.x; it < nst * Sc; it += kBlock) {
-            const int s = it / Sc, bb = it - s * Sc;
-            const T *X = stage + s * stride;
-            const uint8_t *yc = p.ycol + s * ny;
-            const uint64_t bm = dk::unrank((uint32_t)bb, b, mc, C);
-            const uint32_t Sb = (uint32_t)Sc - 1u - (uint32_t)bb;
-            uint32_t Q1 = 0, P1 = 0;
-            int h = 0;
-            double acc = 0;
-            uint64_t rem = bm;
-            while (rem) {
-                const int o = __ffsll((unsigned long long)rem) - 1;
-                rem &= rem - 1;
-                const uint32_t off = (uint32_t)Sm - 1u - Q1 - (Sb - P1 - C(b - 1 - o, mc - h));
-                const double *Ko = K + (a + o) * L;
-                if (ok(off, (uint32_t)Sm))
-                    for (int yi = 0; yi < ny; ++yi) acc += Ko[yc[yi]] * (double)X[yi * Sm + off];
-                Q1 += C(b - 1 - o, mc - 1 - h);
-                P1 += C(b - 1 - o, mc - h);
-                ++h;
-            }
-            const uint32_t zi = p.ctr_zb[s] + (uint32_t)bb;
-            if (ok(zi, ztot)) Z[zi] = (T)acc;
+    if (has_ctr) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < nst * Sc; e += kBlock) {
+            const int s = e / Sc;
+            const uint32_t zi = p.ctr_zb[s] + (uint32_t)(e - s * Sc);
+            if (ok(zi, ztot)) Z[zi] = (T)p.acc[e];
         }
+    }
 }
 
 template <typename T, bool kChecked>
@@ -266,11 +283,14 @@ __global__ __launch_bounds__(kBlock) void k_star_main(dk_tables t, dk_stars st,
         atomicAdd(bad, 1u);
         return false;
     };
-    double *K = lds_raw;
-    uint32_t *bt = (uint32_t *)((char *)lds_raw + a8(sizeof(double) * L * L));
-    T *own = (T *)((char *)bt + a8(sizeof(uint32_t) * (L + 1) * (N + 2)));
-    for (int i = threadIdx.x; i < L * L; i += kBlock) K[i] = t.K[i];
-    for (int i = threadIdx.x; i < (L + 1) * (N + 2); i += kBlock) bt[i] = t.binom[i];
+    double *Kbb = lds_raw; // Kbb[o * b + t] = K between B orbitals o and t
+    uint32_t *bt = (uint32_t *)((char *)lds_raw + a8(sizeof(double) * b * b));
+    T *own = (T *)((char *)bt + a8(sizeof(uint32_t) * (b + 1) * (N + 2)));
+    for (int i = threadIdx.x; i < b * b; i += kBlock) {
+        const int o = i / b, tt = i - o * b;
+        Kbb[i] = t.K[(a + o) * L + a + tt];
+    }
+    for (int i = threadIdx.x; i < (b + 1) * (N + 2); i += kBlock) bt[i] = t.binom[i];
     const uint32_t first = st.group[2 * g], w = st.group[2 * g + 1];
     const int nsl = (int)(w & 0xffffu), m = (int)(w >> 16), k = N - m;
     __syncthreads();
@@ -288,10 +308,15 @@ __global__ __launch_bounds__(kBlock) void k_star_main(dk_tables t, dk_stars st,
         if (!ok(I, n)) continue;
         const T *os = own + s * S;
         double acc = (D[I] - shift) * (double)os[bi];
-        const uint64_t bm = dk::unrank((uint32_t)bi, b, m, C);
-        dk::for_each_hop((uint32_t)bi, bm, b, m, C, [&](int o, int tt, uint32_t J) {
-            if (ok(J, (uint32_t)S)) acc += K[(a + o) * L + a + tt] * (double)os[J];
-        });
+        // hops inside B from the hop table (coalesced across lanes)
+        const uint32_t *hp = st.hop + st.hop_off[m] + bi;
+        const int nh = m * (b - m);
+#pragma unroll 4
+        for (int h = 0; h < nh; ++h) {
+            const uint32_t hv = hp[(size_t)h * S];
+            const uint32_t J = hv & 0xffffu, kx = hv >> 16;
+            if (ok(J, (uint32_t)S) && ok(kx, (uint32_t)(b * b))) acc += Kbb[kx] * (double)os[J];
+        }
         const uint32_t zb = st.slab_zbase[first + s] + (uint32_t)bi;
         double zs = 0;
         for (int q = 0; q < ns; ++q) {
@@ -307,18 +332,39 @@ int launch(const dk_tables *t, const dk_stars *st, const double *D, double shift
            T *sigma, T *Z, uint32_t *bad, hipStream_t s)
 {
     const size_t ls = star_lds<T>(t->L, t->N, *st, nullptr, nullptr);
-    if (st->nsgroup == 0)
-        ; /* a = 0: no stars, every hop is inside the one slab */
-    else if (st->b <= 16 && st->ymax <= 32)
-        hipLaunchKernelGGL((k_star<T, kChecked, 32, 16>), dim3(st->nsgroup), dim3(kBlock), ls, s,
-                           *t, *st, c, Z, bad);
-    else if (st->b <= 16 && st->ymax <= 64)
-        hipLaunchKernelGGL((k_star<T, kChecked, 64, 16>), dim3(st->nsgroup), dim3(kBlock), ls, s,
-                           *t, *st, c, Z, bad);
-    else
-        return (int)hipErrorInvalidValue;
-    int e = (int)hipGetLastError();
-    if (e) return e;
+    if (st->b > 16 || st->a > 64) return (int)hipErrorInvalidValue;
+    // one launch per run of j sectors sharing a bucket YMAX = 8 * ceil((a - j) / 8)
+    const int a = (int)st->a, N = (int)t->N;
+    int j = 0;
+    while (j < N) {
+        const int bucket = (a - j + 7) / 8;
+        int j1 = j + 1;
+        while (j1 < N && (a - j1 + 7) / 8 == bucket) ++j1;
+        const uint32_t g0 = st->jgroup[j], g1 = st->jgroup[j1];
+        j = j1;
+        if (g1 <= g0) continue;
+        const dim3 grid(g1 - g0);
+        switch (bucket) {
+#define DK_STAR_CASE(B)                                                                            \
+    case B:                                                                                        \
+        hipLaunchKernelGGL((k_star<T, kChecked, 8 * B, 16>), grid, dim3(kBlock), ls, s, *t, *st,  \
+                           g0, c, Z, bad);                                                         \
+        break;
+            DK_STAR_CASE(1)
+            DK_STAR_CASE(2)
+            DK_STAR_CASE(3)
+            DK_STAR_CASE(4)
+            DK_STAR_CASE(5)
+            DK_STAR_CASE(6)
+            DK_STAR_CASE(7)
+            DK_STAR_CASE(8)
+#undef DK_STAR_CASE
+        default:
+            return (int)hipErrorInvalidValue;
+        }
+        const int e = (int)hipGetLastError();
+        if (e) return e;
+    }
     const size_t lm = main_lds<T>(t->L, t->N, *st);
     hipLaunchKernelGGL((k_star_main<T, kChecked>), dim3(st->ngroup), dim3(kBlock), lm, s, *t, *st,
                        D, shift, c, (const T *)Z, sigma, bad);

@@ -115,10 +115,12 @@ int doci_op_sigma_dev(doci_op *op, const void *in, void *out, int prec, double s
 /* The operator's HIP stream (hipStream_t) and device. */
 void *doci_op_stream(doci_op *op);
 
-/* Sigma kernel of the operator: 0 = two-pass star kernel (default), 1 = one
- * lane per determinant, 2 = star kernel with every computed index
- * bounds-checked (violations counted, see doci_op_check_violations, and the
- * access skipped), 3 = one-pass slab kernel, 4 = slab kernel bounds-checked. */
+/* Sigma kernel of the operator: 0 = two-pass star kernel (the default;
+ * it sums a centre's hops with LDS atomics, so results may differ between
+ * runs in the last bits), 1 = one lane per determinant, 2 = star kernel with
+ * every computed index bounds-checked (violations counted, see
+ * doci_op_check_violations, and the access skipped), 3 = one-pass slab
+ * kernel (bit-reproducible), 4 = slab kernel bounds-checked. */
 int doci_op_set_kernel(doci_op *op, int kind);
 /* The star split: a low / b high orbitals, main-pass and star-pass
  * workgroups, and the number of entries of the slot buffer. */
@@ -128,9 +130,13 @@ int doci_op_star_split(const doci_op *op, unsigned *a, unsigned *b, unsigned *ng
 int doci_op_slab_split(const doci_op *op, unsigned *a, unsigned *b, unsigned *ngroups);
 /* Index violations counted by kernel kinds 2 and 4 since the operator was built. */
 int64_t doci_op_check_violations(doci_op *op);
-/* Tuning knobs of the slab kernel: "xcd_runs" (0: groups dispatched in rank
+/* Tuning knobs.  Slab kernel: "xcd_runs" (0: groups dispatched in rank
  * order, 1: contiguous runs per XCD), "emax" (determinants staged per
- * workgroup, rebuilds the slab split), "gmax" (slabs per workgroup). */
+ * workgroup, rebuilds the slab split), "gmax" (slabs per workgroup).  Star
+ * kernel (each rebuilds the star tables; 0 restores the default):
+ * "star_lds" (LDS bytes of a star-pass workgroup), "star_sgmax" (stars per
+ * star-pass workgroup), "star_emax" (determinants staged per main-pass
+ * workgroup). */
 int doci_op_tune(doci_op *op, const char *key, int value);
 /* Average event-timed duration (ms) of `reps` sigma launches on scratch
  * vectors of precision prec (0 FP64, 1 FP32). */

@@ -36,7 +36,10 @@ def _checked_case(op, seed=3):
         assert op.check_violations() == 0, kern
         op.set_kernel(kern)
         s = op.sigma(c)
-        assert np.array_equal(s_chk, s), kern
+        if kern == "slab":
+            assert np.array_equal(s_chk, s), kern
+        else:  # the star pass sums a centre's hops with LDS atomics: order varies
+            assert rel_err(s, s_chk) <= 1e-14, kern
         assert rel_err(s, s_lane) <= SIGMA_RTOL, kern
     op.set_kernel("star")
     return c, s_lane
@@ -67,7 +70,11 @@ def test_blocked_kernels_bounds_then_parity(name, fcidump_dir, syn24, tmp_path_f
             s32 = _sigma_fp32(op, c5)
             assert op.check_violations() == 0, kern
             op.set_kernel(kern)
-            assert np.array_equal(_sigma_fp32(op, c5), s32)
+            s = _sigma_fp32(op, c5)
+            if kern == "slab":
+                assert np.array_equal(s, s32)
+            else:
+                assert rel_err(s, s32) <= 1e-6, kern
         op.set_kernel("star")
 
 
