@@ -54,6 +54,17 @@ def cpu_baseline(op, plain_its: int, threads: int) -> dict:
                       f"{per_it:.2f} s/iteration) x {plain_its} iterations of the FP64 solve"}
 
 
+def measured_traffic(op, prec: str):
+    """HBM bytes per sigma call of the kernel in use, from the committed
+    rocprof PMC summary (FETCH_SIZE + WRITE_SIZE scaled by a 1 GiB copy), or
+    None when no summary covers this kernel and shape."""
+    path = os.path.join(ROOT, "profiles", "sigma_star_L34N10_traffic.json")
+    if op.kernel != "star" or (op.L, op.N) != (34, 10) or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return round(json.load(f)["hbm_bytes_per_sigma"][prec] / 1e9, 2)
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -148,7 +159,10 @@ def main() -> int:
         "sigma_ms": {"fp32": round(1e3 * avg, 3), "fp64": round(1e3 * avg64, 3)},
         "roofline": {"bound": "hbm", "kernel": kern, "achieved": round(achieved / 1e9, 1),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK, 4), "traffic": None},
+                     "frac": round(achieved / HBM_PEAK, 4),
+                     "traffic": measured_traffic(op, "fp32" if "FP32" in kern else "fp64"),
+                     "traffic_unit": "GB per sigma call (rocprof FETCH_SIZE + WRITE_SIZE, calibrated)",
+                     "algorithmic_GB": round(bytes_per / 1e9, 3)},
     }
     if not args.no_cpu:
         log("cpu baseline")

@@ -205,6 +205,7 @@ class Operator:
         self.n = int(lib().doci_op_dim(handle))
         self.L = int(lib().doci_op_L(handle))
         self.N = int(lib().doci_op_N(handle))
+        self.kernel = "star"  # the library's default sigma kernel
 
     @classmethod
     def from_fcidump(cls, path: str) -> "Operator":
@@ -252,6 +253,7 @@ class Operator:
         """Sigma kernel: "star" (default), "slab" (bit-reproducible), "lane", or a
         "-checked" variant."""
         _check(lib().doci_op_set_kernel(self._h, self.KERNELS[name]), "doci_op_set_kernel")
+        self.kernel = name
 
     def slab_split(self):
         a, b, g = C.c_uint(), C.c_uint(), C.c_uint()
