@@ -365,3 +365,28 @@ def test_cli_end_to_end(fcidump_dir, golden, tmp_path):
     assert abs(float(line.split(":")[1]) - golden["logs"]["lifHF"]["energy"]) < 1e-9
     assert "EXIT CODE SOLVER : 0" in r.stdout
     assert "DAVIDSON ITERATIONS : 10" in r.stdout
+
+
+REF_MAIN = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                        "oracle", "_ref", "doci_main_on_amd")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_MAIN),
+                    reason="reference main() not linked here (needs the reference sources at build time)")
+def test_reference_main_linked_against_library(fcidump_dir, golden, tmp_path):
+    """The reference's own executable.c + macros.c, linked against
+    libdoci_amd.so instead of its operator and solver sources, solves the CO
+    example of tests/DOCI_input.in and prints the reference log's energy and
+    iteration count (tests/coHF.out:38-43)."""
+    print("[case] reference main() on the library: co_631g_1", flush=True)
+    (tmp_path / "DOCI_input.in").write_text(
+        "FCIDUMP= co_631g_1.FCIDUMP\nSOLVER = D\nMAX_ITS=1000\nTOL=1e-8\n"
+        "DAVIDSON_KEEP=\nDAVIDSON_MAX_VEC=\nHF_INIT= 1\n")
+    (tmp_path / "co_631g_1.FCIDUMP").write_bytes((fcidump_dir / "co_631g_1.FCIDUMP").read_bytes())
+    r = subprocess.run([REF_MAIN, "DOCI_input.in"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    line = [l for l in r.stdout.splitlines() if "DOCI ENERGY" in l][0]
+    assert abs(float(line.split(":")[1]) - golden["logs"]["coHF"]["energy"]) < 1e-9
+    assert "EXIT CODE SOLVER : 0" in r.stdout
+    assert f"DAVIDSON ITERATIONS : {golden['logs']['coHF']['iterations']}" in r.stdout
