@@ -42,6 +42,18 @@ class SolveInfo(C.Structure):
                 ("seconds_sigma_fp32", C.c_double)]
 
 
+class BatchOpts(C.Structure):
+    _fields_ = [("tol", C.c_double), ("max_its", C.c_int), ("keep", C.c_int),
+                ("max_vec", C.c_int), ("switch_tol", C.c_double), ("use_graphs", C.c_int),
+                ("tile_rows", C.c_int), ("mem_budget", C.c_size_t), ("max_instances", C.c_int)]
+
+
+class BatchResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("iterations", C.c_int), ("iterations_fp32", C.c_int),
+                ("energy", C.c_double), ("residual", C.c_double), ("t_submit", C.c_double),
+                ("t_start", C.c_double), ("t_done", C.c_double)]
+
+
 class InputFile(C.Structure):
     _fields_ = [("fcidump", C.c_char * 255), ("solver", C.c_char * 20), ("max_its", C.c_int),
                 ("tol", C.c_double), ("davidson_keep", C.c_int),
@@ -58,7 +70,9 @@ EXPORTS = [
     "doci_write_fcidump", "doci_gen_integrals", "doci_gen_vector", "doci_read_inputfile",
     "doci_read_fcidump", "doci_free", "doci_dev_copy", "doci_op_set_kernel",
     "doci_op_slab_split", "doci_op_check_violations", "doci_op_tune", "doci_op_time_sigma",
-    "doci_op_star_split",
+    "doci_op_star_split", "doci_batch_defaults", "doci_batch_create", "doci_batch_free",
+    "doci_batch_footprint", "doci_batch_add", "doci_batch_step", "doci_batch_run",
+    "doci_batch_result_of", "doci_batch_release", "doci_batch_clock", "doci_batch_stats",
 ]
 
 
@@ -109,6 +123,20 @@ def lib():
                                      C.POINTER(C.c_uint), C.POINTER(C.c_uint),
                                      C.POINTER(C.c_size_t)]
     l.doci_op_time_sigma.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    l.doci_batch_defaults.argtypes = [C.POINTER(BatchOpts)]
+    l.doci_batch_create.argtypes = [C.POINTER(BatchOpts), C.POINTER(vp)]
+    l.doci_batch_free.argtypes = [vp]
+    l.doci_batch_footprint.argtypes = [C.POINTER(BatchOpts), C.c_uint, C.c_uint]
+    l.doci_batch_footprint.restype = C.c_size_t
+    l.doci_batch_add.argtypes = [vp, C.c_uint, C.c_uint, C.c_double, _dp, _dp, C.c_int,
+                                 C.POINTER(C.c_int)]
+    l.doci_batch_step.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    l.doci_batch_run.argtypes = [vp]
+    l.doci_batch_result_of.argtypes = [vp, C.c_int, C.POINTER(BatchResult), vp]
+    l.doci_batch_release.argtypes = [vp, C.c_int]
+    l.doci_batch_clock.argtypes = [vp]
+    l.doci_batch_clock.restype = C.c_double
+    l.doci_batch_stats.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int)]
     _lib = l
     return l
 
@@ -304,6 +332,93 @@ class Operator:
     def close(self):
         if self._h:
             lib().doci_op_free(self._h)
+            self._h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
+# many independent solves (lock-step batch / continuous-batching serving)
+# ---------------------------------------------------------------------------
+
+class Batch:
+    """Independent DOCI instances solved together on one GPU (see
+    doci_batch_* in include/doci_amd.h).  add() queues an instance, step()
+    admits what fits and runs one Davidson iteration of every running
+    instance, run() steps until all queued instances have finished."""
+
+    STATUS = {0: "waiting", 1: "running", 2: "converged", 3: "max_its", 4: "breakdown"}
+
+    def __init__(self, tol=1e-8, max_its=200, keep=2, max_vec=30, switch_tol=1e-5,
+                 use_graphs=True, tile_rows=0, mem_budget=0, max_instances=1024):
+        l = lib()
+        o = BatchOpts()
+        l.doci_batch_defaults(C.byref(o))
+        o.tol, o.max_its, o.keep, o.max_vec, o.switch_tol = tol, max_its, keep, max_vec, switch_tol
+        o.use_graphs, o.tile_rows, o.mem_budget = int(use_graphs), tile_rows, mem_budget
+        o.max_instances = max_instances
+        self.opts = o
+        h = C.c_void_p()
+        _check(l.doci_batch_create(C.byref(o), C.byref(h)), "doci_batch_create")
+        self._h = h
+        self.dims = {}
+
+    def footprint(self, L: int, N: int) -> int:
+        return int(lib().doci_batch_footprint(C.byref(self.opts), L, N))
+
+    def add(self, L: int, N: int, core: float, one, two, precision: str = "fp64") -> int:
+        i = C.c_int()
+        _check(lib().doci_batch_add(self._h, L, N, core, np.ascontiguousarray(one, np.float64),
+                                    np.ascontiguousarray(two, np.float64),
+                                    1 if precision == "mixed" else 0, C.byref(i)),
+               "doci_batch_add")
+        self.dims[i.value] = binom(N, L)
+        return i.value
+
+    def add_synthetic(self, L: int, N: int, seed: int, precision: str = "fp64") -> int:
+        core, one, two = gen_integrals(L, seed)
+        return self.add(L, N, core, one, two, precision)
+
+    def step(self):
+        r, w = C.c_int(), C.c_int()
+        _check(lib().doci_batch_step(self._h, C.byref(r), C.byref(w)), "doci_batch_step")
+        return r.value, w.value
+
+    def run(self) -> None:
+        _check(lib().doci_batch_run(self._h), "doci_batch_run")
+
+    def result(self, i: int, want_x: bool = False):
+        r = BatchResult()
+        x = np.zeros(self.dims[i]) if want_x else None
+        _check(lib().doci_batch_result_of(self._h, i, C.byref(r),
+                                          None if x is None else x.ctypes.data),
+               "doci_batch_result_of")
+        return r, x
+
+    def release(self, i: int) -> None:
+        _check(lib().doci_batch_release(self._h, i), "doci_batch_release")
+
+    def clock(self) -> float:
+        return float(lib().doci_batch_clock(self._h))
+
+    def stats(self):
+        t, it, g = C.c_int(), C.c_int(), C.c_int()
+        lib().doci_batch_stats(self._h, C.byref(t), C.byref(it), C.byref(g))
+        return dict(tile_rows=t.value, iterations=it.value, graph_launches=g.value)
+
+    def close(self):
+        if self._h:
+            lib().doci_batch_free(self._h)
             self._h = None
 
     def __enter__(self):

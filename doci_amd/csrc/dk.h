@@ -114,6 +114,82 @@ size_t dk_sigma_star_lds(const dk_tables *t, const dk_stars *st, int prec, int p
 int dk_sigma_star(const dk_tables *t, const dk_stars *st, const double *D, double shift,
                   const void *c, void *sigma, void *Z, int prec, uint32_t *bad, hipStream_t s);
 
+/* ---- batched multi-instance Davidson -----------------------------------
+ * Many independent DOCI instances iterate in lock step; every launch covers
+ * all of them.  Streaming launches run one workgroup per tile (a fixed-size
+ * row block of one instance, listed in `tiles`) and write one row of
+ * DK_BPV partial sums per tile; "small" launches run one 64-lane workgroup
+ * per instance, reduce that instance's tile partials in a fixed order and
+ * advance its Davidson state (dk_bstate, device resident).  Each instance's
+ * vectors live in slots of `slot_stride` bytes: V slots [0, P) and W slots
+ * [P, 2P), P = max_vec + keep. */
+#define DK_BMAX_VEC 32
+#define DK_BKEEP_MAX 4
+#define DK_BPV (DK_BMAX_VEC + 2)
+
+typedef struct {             /* constant per instance */
+    uint32_t L, N, n;
+    uint32_t tile0, ntiles;  /* its tiles in the tile list */
+    uint32_t lane_sigma;     /* 1: sigma by the batched lane kernel */
+    const uint32_t *binom;   /* (L+1) x (N+2) */
+    const double *K;         /* L x L, zero diagonal */
+    const double *D;         /* diagonal, n */
+    char *slots;
+    uint64_t slot_stride;
+    double *x;               /* FP64 Ritz vector, n */
+} dk_binst;
+
+enum { DK_B_IDLE = 0, DK_B_ACTIVE = 1, DK_B_CONVERGED = 2, DK_B_MAXITS = 3, DK_B_BREAKDOWN = 4 };
+enum { DK_BF_FIRST = 1, DK_BF_ORTHO2 = 2, DK_BF_DEFLATE = 4 };
+
+typedef struct {             /* summary copied to the host every iteration */
+    int32_t status, m, its, flags;
+    double energy, res;
+} dk_bstat;
+
+typedef struct {             /* mutable per instance */
+    dk_bstat st;
+    int32_t max_its, keep, max_vec, P;
+    int32_t tslot;           /* V slot holding t (the next basis vector) */
+    int32_t sin, sout;       /* sigma input (V slot) and output (W slot) */
+    int32_t vslot[DK_BMAX_VEC], wslot[DK_BMAX_VEC];
+    int32_t dv[DK_BKEEP_MAX], dw[DK_BKEEP_MAX]; /* deflation targets */
+    uint64_t vused, wused;   /* slot occupancy bitmasks */
+    double tol, otol, shift, theta;  /* theta: lowest eigenvalue of V^T (H - shift) V */
+    double s[DK_BMAX_VEC];   /* V_j = s_j * stored V_j (same for W_j) */
+    double g[DK_BMAX_VEC];   /* ortho coefficients on stored V */
+    double y[DK_BMAX_VEC];   /* residual coefficients on stored V / W */
+    double defl[DK_BKEEP_MAX][DK_BMAX_VEC];
+    double A[DK_BMAX_VEC * DK_BMAX_VEC]; /* projected matrix in the current eigenbasis */
+    double Y[DK_BMAX_VEC * DK_BMAX_VEC]; /* eigenvectors, column k = Y[j*DK_BMAX_VEC+k] */
+    const void *sig_c;       /* star-kernel sigma of this instance: in / out / shift */
+    void *sig_out;
+    double sig_shift;
+} dk_bstate;
+
+enum { DK_BPH_A = 0, DK_BPH_B = 1, DK_BPH_C = 2, DK_BPH_D = 3, DK_BPH_E = 4 };
+enum { DK_BOP_ORTHO = 0, DK_BOP_ORTHO2 = 1, DK_BOP_SCOL = 2, DK_BOP_DEFLATE = 3,
+       DK_BOP_RESID = 4 };
+
+/* Streaming step `op` over all tiles; M = largest basis size it may meet
+ * (picks the instantiation).  part: ntile x DK_BPV doubles. */
+int dk_batch_stream(int op, int prec, int M, const dk_binst *inst, dk_bstate *state,
+                    const uint32_t *tiles, uint32_t ntile, uint32_t tile_rows, double *part,
+                    hipStream_t s);
+/* Small step `phase` for the ninst instances listed in ilist (device). */
+int dk_batch_small(int phase, const uint32_t *ilist, uint32_t ninst, const dk_binst *inst,
+                   dk_bstate *state, const double *part, hipStream_t s);
+/* sigma of every active lane-sigma instance: W[sout] = (H - shift) V[sin]. */
+int dk_batch_sigma_lane(int prec, uint32_t maxL, uint32_t maxN, const dk_binst *inst,
+                        dk_bstate *state, const uint32_t *tiles, uint32_t ntile,
+                        uint32_t tile_rows, hipStream_t s);
+/* Admission of instance i (FP64 x, D on device): t = e_argmin(D) (or t = x
+ * when from_x), state reset for precision prec; shift = D[argmin] when
+ * shift_min_d (else 0).  red: dk_red_scratch_doubles(1) doubles. */
+int dk_batch_start(const dk_binst *inst_host, dk_bstate *state_i, int from_x, int prec,
+                   int shift_min_d, double tol, double otol, int max_its, int keep, int max_vec,
+                   int its0, double *red, int64_t *idx, hipStream_t s);
+
 /* ---- streaming vector kernels (deterministic two-level reductions) -------
  * Vectors are arrays of n elements; `vecs` lists device pointers.  prec as
  * above for the stored vectors; reductions and coefficients are FP64.

@@ -1,0 +1,652 @@
+/* batch.c -- many independent DOCI Davidson solves on one GPU.
+ *
+ * The reference solves one problem per process (src/executable.c:41-90).
+ * Here a batch holds any number of instances of any (L, N) that run the
+ * Davidson iteration of src/davidson.c:72-99 in lock step: one iteration
+ * is a fixed sequence of launches over all running instances (kernels in
+ * batch.hip), recorded once in a HIP graph and replayed while the set of
+ * running instances and the basis-size bucket stay the same.  Instances can
+ * be queued at any time; each step first admits queued instances in FIFO
+ * order while their device footprint fits the memory budget, runs one
+ * iteration, then retires finished ones (continuous batching).
+ *
+ * Mixed-precision instances run an FP32-storage phase (operator shifted by
+ * min D, stopping at switch_tol) and then restart in FP64 from their Ritz
+ * vector, exactly like the single-instance mixed solve; the two phases are
+ * separate launch groups, so FP32 and FP64 instances run side by side.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "internal.h"
+
+typedef struct {
+    int state;              /* 0 queued, 1 running, 2 finished */
+    int want_mixed;
+    int prec;               /* running phase: 1 FP32 storage, 0 FP64 */
+    int slot;               /* device instance slot while running, else -1 */
+    unsigned L, N;
+    uint32_t n;
+    double core;
+    double *one, *two;      /* raw integrals until admission */
+    void *dmain;            /* tables, D and vector slots */
+    double *dx;             /* FP64 Ritz vector (kept until release) */
+    size_t bytes;           /* device bytes of dmain */
+    dk_binst bi;
+    doci_batch_result res;
+} brec;
+
+typedef struct {
+    int n;                  /* running instances of this precision */
+    uint32_t *h_list, *d_list;
+    size_t list_cap;
+    uint32_t ntile;
+    uint32_t *h_tiles, *d_tiles;
+    size_t tiles_cap;
+    double *d_part;
+    size_t part_cap;
+    hipGraph_t graph;
+    hipGraphExec_t gexec;
+    int gM;                 /* bucket of the captured graph, 0 = none */
+    uint32_t maxL, maxN;
+} bgroup;
+
+struct doci_batch {
+    doci_batch_opts o;
+    hipStream_t stream;
+    int cap;
+    dk_binst *d_inst;
+    dk_bstate *d_state;
+    dk_bstat *h_stat;       /* pinned */
+    int *slot_rec;          /* device slot -> record, -1 free */
+    brec *rec;
+    int nrec, rec_cap;
+    int qhead;              /* first record that may still be queued */
+    bgroup grp[2];          /* [0] FP64, [1] FP32 storage */
+    int dirty;
+    size_t mem_used;
+    double t0;
+    int tile_rows, tuned;
+    int iterations, graph_launches;
+    double *d_red;
+    int64_t *d_idx;
+};
+
+#define ALIGN256(x) (((size_t)(x) + 255) & ~(size_t)255)
+
+void doci_batch_defaults(doci_batch_opts *o)
+{
+    memset(o, 0, sizeof *o);
+    o->tol = 1e-8;
+    o->max_its = 200;
+    o->keep = 2;
+    o->max_vec = 30;
+    o->switch_tol = 1e-5;
+    o->use_graphs = 1;
+    o->tile_rows = 0;
+    o->mem_budget = 0;
+    o->max_instances = 1024;
+}
+
+static size_t table_bytes(unsigned L, unsigned N)
+{
+    const size_t l = L;
+    return ALIGN256(sizeof(double) * (3 * l * l + l) + sizeof(uint32_t) * (l + 1) * (N + 2));
+}
+
+static size_t slot_stride(uint32_t n) { return ALIGN256(sizeof(double) * (size_t)n); }
+
+static size_t main_bytes(const doci_batch_opts *o, unsigned L, unsigned N, uint32_t n)
+{
+    const size_t P = (size_t)(o->max_vec + o->keep);
+    return table_bytes(L, N) + slot_stride(n) /* D */ + 2 * P * slot_stride(n);
+}
+
+size_t doci_batch_footprint(const doci_batch_opts *o, unsigned L, unsigned N)
+{
+    const uint64_t n = dh_binom(N, L);
+    return main_bytes(o, L, N, (uint32_t)n) + slot_stride((uint32_t)n);
+}
+
+double doci_batch_clock(const doci_batch *b) { return dh_now() - b->t0; }
+
+int doci_batch_create(const doci_batch_opts *o_in, doci_batch **out)
+{
+    *out = NULL;
+    doci_batch_opts o;
+    if (o_in)
+        o = *o_in;
+    else
+        doci_batch_defaults(&o);
+    if (o.max_vec < 2 || o.max_vec > DK_BMAX_VEC - 2 || o.keep < 1 || o.keep > DK_BKEEP_MAX ||
+        o.keep >= o.max_vec || o.max_instances < 1 || !(o.tol > 0) || o.max_its < 0)
+        return doci_fail(DOCI_E_ARG, "batch options out of range (max_vec 2..%d, keep 1..%d)",
+                         DK_BMAX_VEC - 2, DK_BKEEP_MAX);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+        return doci_fail(DOCI_E_HIP, "no HIP device available");
+    doci_batch *b = calloc(1, sizeof *b);
+    if (!b) return doci_fail(DOCI_E_NOMEM, "batch");
+    b->o = o;
+    b->cap = o.max_instances;
+    b->t0 = dh_now();
+    b->tile_rows = o.tile_rows;
+    b->tuned = o.tile_rows > 0;
+    int rc = DOCI_OK;
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_inst, sizeof(dk_binst) * b->cap);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_state, sizeof(dk_bstate) * b->cap);
+    if (e == hipSuccess) e = hipMemset(b->d_state, 0, sizeof(dk_bstate) * b->cap);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_stat, sizeof(dk_bstat) * b->cap, 0);
+    if (e == hipSuccess) memset(b->h_stat, 0, sizeof(dk_bstat) * b->cap);
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_red, sizeof(double) * dk_red_scratch_doubles(4));
+    if (e == hipSuccess) e = hipMalloc((void **)&b->d_idx, sizeof(int64_t));
+    if (e != hipSuccess) rc = doci_fail(DOCI_E_HIP, "batch setup: %s", hipGetErrorString(e));
+    b->slot_rec = malloc(sizeof(int) * b->cap);
+    if (rc == DOCI_OK && !b->slot_rec) rc = doci_fail(DOCI_E_NOMEM, "batch");
+    if (rc != DOCI_OK) {
+        doci_batch_free(b);
+        return rc;
+    }
+    for (int i = 0; i < b->cap; ++i) b->slot_rec[i] = -1;
+    if (!o.mem_budget) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess) b->o.mem_budget = (size_t)(0.9 * (double)fr);
+    }
+    *out = b;
+    return DOCI_OK;
+}
+
+static void group_free(bgroup *g)
+{
+    if (g->gexec) hipGraphExecDestroy(g->gexec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    if (g->d_list) hipFree(g->d_list);
+    if (g->d_tiles) hipFree(g->d_tiles);
+    if (g->d_part) hipFree(g->d_part);
+    free(g->h_list);
+    free(g->h_tiles);
+    memset(g, 0, sizeof *g);
+}
+
+static void rec_free_main(doci_batch *b, brec *r)
+{
+    if (r->dmain) {
+        hipFree(r->dmain);
+        r->dmain = NULL;
+        b->mem_used -= r->bytes;
+    }
+}
+
+void doci_batch_free(doci_batch *b)
+{
+    if (!b) return;
+    if (b->stream) hipStreamSynchronize(b->stream);
+    for (int i = 0; i < b->nrec; ++i) {
+        brec *r = &b->rec[i];
+        rec_free_main(b, r);
+        if (r->dx) hipFree(r->dx);
+        free(r->one);
+        free(r->two);
+    }
+    free(b->rec);
+    for (int g = 0; g < 2; ++g) group_free(&b->grp[g]);
+    if (b->d_inst) hipFree(b->d_inst);
+    if (b->d_state) hipFree(b->d_state);
+    if (b->h_stat) hipHostFree(b->h_stat);
+    if (b->d_red) hipFree(b->d_red);
+    if (b->d_idx) hipFree(b->d_idx);
+    if (b->stream) hipStreamDestroy(b->stream);
+    free(b->slot_rec);
+    free(b);
+}
+
+int doci_batch_add(doci_batch *b, unsigned L, unsigned N, double core, const double *one,
+                   const double *two, int precision, int *id)
+{
+    if (L == 0 || L > DK_MAX_L) return doci_fail(DOCI_E_ORBITALS, "Too many orbitals (L=%u).", L);
+    if (N == 0 || N > L) return doci_fail(DOCI_E_DIM, "invalid number of pairs N=%u", N);
+    const uint64_t n = dh_binom(N, L);
+    if (n >= (1ull << 31))
+        return doci_fail(DOCI_E_DIM, "C(%u,%u) = %llu exceeds 2^31-1", L, N, (unsigned long long)n);
+    if (doci_batch_footprint(&b->o, L, N) > b->o.mem_budget)
+        return doci_fail(DOCI_E_NOMEM, "instance L=%u N=%u needs more than the memory budget", L, N);
+    if (b->nrec == b->rec_cap) {
+        const int nc = b->rec_cap ? 2 * b->rec_cap : 64;
+        brec *nr = realloc(b->rec, sizeof(brec) * nc);
+        if (!nr) return doci_fail(DOCI_E_NOMEM, "batch records");
+        b->rec = nr;
+        b->rec_cap = nc;
+    }
+    brec *r = &b->rec[b->nrec];
+    memset(r, 0, sizeof *r);
+    const size_t l2 = (size_t)L * L;
+    r->one = malloc(sizeof(double) * l2);
+    r->two = malloc(sizeof(double) * l2 * l2);
+    if (!r->one || !r->two) {
+        free(r->one);
+        free(r->two);
+        return doci_fail(DOCI_E_NOMEM, "instance integrals");
+    }
+    memcpy(r->one, one, sizeof(double) * l2);
+    memcpy(r->two, two, sizeof(double) * l2 * l2);
+    r->L = L;
+    r->N = N;
+    r->n = (uint32_t)n;
+    r->core = core;
+    r->want_mixed = precision != 0;
+    r->slot = -1;
+    r->res.t_submit = doci_batch_clock(b);
+    *id = b->nrec++;
+    return DOCI_OK;
+}
+
+/* Builds the instance's tables and diagonal on the device and starts its
+ * first phase in device slot `slot`. */
+static int admit(doci_batch *b, int ri, int slot)
+{
+    brec *r = &b->rec[ri];
+    const unsigned L = r->L, N = r->N;
+    const uint32_t n = r->n;
+    const size_t l = L, l2 = l * l, tb = table_bytes(L, N), vs = slot_stride(n);
+    double *K = malloc(sizeof(double) * l2), *J = malloc(sizeof(double) * l2),
+           *X = malloc(sizeof(double) * l2), *h = malloc(sizeof(double) * l);
+    uint32_t *bt = malloc(sizeof(uint32_t) * (l + 1) * (N + 2));
+    int rc = DOCI_OK;
+    if (!K || !J || !X || !h || !bt) rc = doci_fail(DOCI_E_NOMEM, "instance tables");
+    if (rc == DOCI_OK && dh_build_tables(L, N, r->one, r->two, K, J, X, h, bt) != DOCI_OK)
+        rc = doci_fail(DOCI_E_NOMEM, "instance tables");
+    r->bytes = main_bytes(&b->o, L, N, n);
+    hipError_t e = hipSuccess;
+    if (rc == DOCI_OK) {
+        e = hipMalloc(&r->dmain, r->bytes);
+        if (e == hipSuccess) e = hipMalloc((void **)&r->dx, sizeof(double) * (size_t)n);
+        if (e != hipSuccess) {
+            if (r->dmain) hipFree(r->dmain);
+            r->dmain = NULL;
+            rc = doci_fail(DOCI_E_NOMEM, "instance memory: %s", hipGetErrorString(e));
+        }
+    }
+    if (rc == DOCI_OK) {
+        b->mem_used += r->bytes;
+        char *d = r->dmain;
+        double *dK = (double *)d, *dJ = dK + l2, *dX = dJ + l2, *dh = dX + l2;
+        uint32_t *db = (uint32_t *)(dh + l);
+        e = hipMemcpy(dK, K, sizeof(double) * l2, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dJ, J, sizeof(double) * l2, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dX, X, sizeof(double) * l2, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(dh, h, sizeof(double) * l, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(db, bt, sizeof(uint32_t) * (l + 1) * (N + 2), hipMemcpyHostToDevice);
+        dk_tables dt = {L, N, n, r->core, db, dK, dJ, dX, dh};
+        double *D = (double *)(d + tb);
+        if (e == hipSuccess) e = (hipError_t)dk_diagonal(&dt, D, b->stream);
+        dk_binst *bi = &r->bi;
+        memset(bi, 0, sizeof *bi);
+        bi->L = L;
+        bi->N = N;
+        bi->n = n;
+        bi->lane_sigma = 1;
+        bi->binom = db;
+        bi->K = dK;
+        bi->D = D;
+        bi->slots = d + tb + vs;
+        bi->slot_stride = vs;
+        bi->x = r->dx;
+        r->prec = r->want_mixed ? 1 : 0;
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(b->d_inst + slot, bi, sizeof *bi, hipMemcpyHostToDevice, b->stream);
+        if (e == hipSuccess)
+            e = (hipError_t)dk_batch_start(bi, b->d_state + slot, 0, r->prec, r->prec,
+                                           r->prec ? b->o.switch_tol : b->o.tol,
+                                           r->prec ? 1e-6 : 1e-12, b->o.max_its, b->o.keep,
+                                           b->o.max_vec, 0, b->d_red, b->d_idx, b->stream);
+        /* the host copy of bi is read by the async copy: finish it here */
+        if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
+        if (e != hipSuccess) rc = doci_fail(DOCI_E_HIP, "instance start: %s", hipGetErrorString(e));
+    }
+    free(K);
+    free(J);
+    free(X);
+    free(h);
+    free(bt);
+    if (rc != DOCI_OK) return rc;
+    free(r->one);
+    free(r->two);
+    r->one = r->two = NULL;
+    memset(&b->h_stat[slot], 0, sizeof(dk_bstat));
+    r->slot = slot;
+    r->state = 1;
+    r->res.status = 1;
+    r->res.t_start = doci_batch_clock(b);
+    b->slot_rec[slot] = ri;
+    b->dirty = 1;
+    return DOCI_OK;
+}
+
+static int free_slot(const doci_batch *b)
+{
+    for (int s = 0; s < b->cap; ++s)
+        if (b->slot_rec[s] < 0) return s;
+    return -1;
+}
+
+/* FIFO admission while the footprint fits the budget. */
+static int admit_queued(doci_batch *b)
+{
+    while (b->qhead < b->nrec && b->rec[b->qhead].state != 0) ++b->qhead;
+    for (int ri = b->qhead; ri < b->nrec; ++ri) {
+        brec *r = &b->rec[ri];
+        if (r->state != 0) continue;
+        const size_t need = main_bytes(&b->o, r->L, r->N, r->n);
+        if (b->mem_used + need > b->o.mem_budget) break;
+        const int slot = free_slot(b);
+        if (slot < 0) break;
+        const int rc = admit(b, ri, slot);
+        if (rc != DOCI_OK) return rc;
+    }
+    return DOCI_OK;
+}
+
+static int grow(void **h, void **d, size_t *cap, size_t need, size_t elem)
+{
+    if (need <= *cap) return DOCI_OK;
+    size_t nc = *cap ? *cap : 64;
+    while (nc < need) nc *= 2;
+    if (h) {
+        void *nh = realloc(*h, elem * nc);
+        if (!nh) return doci_fail(DOCI_E_NOMEM, "batch lists");
+        *h = nh;
+    }
+    if (*d) hipFree(*d);
+    *d = NULL;
+    if (hipMalloc(d, elem * nc) != hipSuccess) return doci_fail(DOCI_E_NOMEM, "batch lists");
+    *cap = nc;
+    return DOCI_OK;
+}
+
+static void drop_graph(bgroup *g)
+{
+    if (g->gexec) hipGraphExecDestroy(g->gexec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    g->gexec = NULL;
+    g->graph = NULL;
+    g->gM = 0;
+}
+
+/* Instance lists and tile lists of both precision groups. */
+static int rebuild(doci_batch *b)
+{
+    const uint32_t rows = (uint32_t)b->tile_rows;
+    for (int p = 0; p < 2; ++p) {
+        bgroup *g = &b->grp[p];
+        drop_graph(g);
+        int cnt = 0;
+        size_t ntile = 0;
+        g->maxL = g->maxN = 0;
+        for (int s = 0; s < b->cap; ++s) {
+            const int ri = b->slot_rec[s];
+            if (ri < 0 || b->rec[ri].prec != p) continue;
+            ++cnt;
+            ntile += (b->rec[ri].n + rows - 1) / rows;
+        }
+        int rc = grow((void **)&g->h_list, (void **)&g->d_list, &g->list_cap, (size_t)cnt + 1,
+                      sizeof(uint32_t));
+        if (rc == DOCI_OK)
+            rc = grow((void **)&g->h_tiles, (void **)&g->d_tiles, &g->tiles_cap, 2 * ntile + 2,
+                      sizeof(uint32_t));
+        if (rc == DOCI_OK) rc = grow(NULL, (void **)&g->d_part, &g->part_cap, ntile * DK_BPV + 1, sizeof(double));
+        if (rc != DOCI_OK) return rc;
+        uint32_t t = 0;
+        g->n = 0;
+        for (int s = 0; s < b->cap; ++s) {
+            const int ri = b->slot_rec[s];
+            if (ri < 0 || b->rec[ri].prec != p) continue;
+            brec *r = &b->rec[ri];
+            g->h_list[g->n++] = (uint32_t)s;
+            r->bi.tile0 = t;
+            r->bi.ntiles = (r->n + rows - 1) / rows;
+            for (uint32_t k = 0; k < r->bi.ntiles; ++k) {
+                g->h_tiles[2 * (t + k)] = (uint32_t)s;
+                g->h_tiles[2 * (t + k) + 1] = k;
+            }
+            t += r->bi.ntiles;
+            if (r->L > g->maxL) g->maxL = r->L;
+            if (r->N > g->maxN) g->maxN = r->N;
+            HIPCHK(hipMemcpyAsync(b->d_inst + s, &r->bi, sizeof r->bi, hipMemcpyHostToDevice, b->stream));
+        }
+        g->ntile = t;
+        if (g->n) {
+            HIPCHK(hipMemcpyAsync(g->d_list, g->h_list, sizeof(uint32_t) * g->n,
+                                  hipMemcpyHostToDevice, b->stream));
+            HIPCHK(hipMemcpyAsync(g->d_tiles, g->h_tiles, sizeof(uint32_t) * 2 * t,
+                                  hipMemcpyHostToDevice, b->stream));
+        }
+    }
+    /* host arrays are sources of async copies: complete them */
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->dirty = 0;
+    return DOCI_OK;
+}
+
+/* One Davidson iteration of every instance of group p (basis bucket M). */
+static int enqueue_iteration(doci_batch *b, int p, int M)
+{
+    bgroup *g = &b->grp[p];
+    const uint32_t rows = (uint32_t)b->tile_rows;
+    hipStream_t s = b->stream;
+    DKCHK(dk_batch_small(DK_BPH_A, g->d_list, g->n, b->d_inst, b->d_state, g->d_part, s));
+    DKCHK(dk_batch_stream(DK_BOP_ORTHO, p, M, b->d_inst, b->d_state, g->d_tiles, g->ntile, rows, g->d_part, s));
+    DKCHK(dk_batch_small(DK_BPH_B, g->d_list, g->n, b->d_inst, b->d_state, g->d_part, s));
+    DKCHK(dk_batch_stream(DK_BOP_ORTHO2, p, M, b->d_inst, b->d_state, g->d_tiles, g->ntile, rows, g->d_part, s));
+    DKCHK(dk_batch_small(DK_BPH_C, g->d_list, g->n, b->d_inst, b->d_state, g->d_part, s));
+    DKCHK(dk_batch_sigma_lane(p, g->maxL, g->maxN, b->d_inst, b->d_state, g->d_tiles, g->ntile, rows, s));
+    DKCHK(dk_batch_stream(DK_BOP_SCOL, p, M, b->d_inst, b->d_state, g->d_tiles, g->ntile, rows, g->d_part, s));
+    DKCHK(dk_batch_small(DK_BPH_D, g->d_list, g->n, b->d_inst, b->d_state, g->d_part, s));
+    DKCHK(dk_batch_stream(DK_BOP_DEFLATE, p, M, b->d_inst, b->d_state, g->d_tiles, g->ntile, rows, g->d_part, s));
+    DKCHK(dk_batch_small(DK_BPH_E, g->d_list, g->n, b->d_inst, b->d_state, g->d_part, s));
+    DKCHK(dk_batch_stream(DK_BOP_RESID, p, M, b->d_inst, b->d_state, g->d_tiles, g->ntile, rows, g->d_part, s));
+    return DOCI_OK;
+}
+
+static int run_iteration(doci_batch *b, int p, int M)
+{
+    bgroup *g = &b->grp[p];
+    if (!b->o.use_graphs) return enqueue_iteration(b, p, M);
+    if (!g->gexec || g->gM != M) {
+        drop_graph(g);
+        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_iteration(b, p, M);
+        hipGraph_t graph = NULL;
+        const hipError_t e = hipStreamEndCapture(b->stream, &graph);
+        if (rc != DOCI_OK) {
+            if (graph) hipGraphDestroy(graph);
+            return rc;
+        }
+        if (e != hipSuccess) return doci_fail(DOCI_E_HIP, "graph capture: %s", hipGetErrorString(e));
+        g->graph = graph;
+        HIPCHK(hipGraphInstantiate(&g->gexec, graph, NULL, NULL, 0));
+        g->gM = M;
+    }
+    HIPCHK(hipGraphLaunch(g->gexec, b->stream));
+    ++b->graph_launches;
+    return DOCI_OK;
+}
+
+static int bucket_of(int m) { return m <= 8 ? 8 : m <= 16 ? 16 : DK_BMAX_VEC; }
+
+/* Times the batched lane sigma and one streaming pass for each tile size
+ * while every running instance is still at its start (m = 0): the sigma
+ * then writes only a W slot not yet in use and the pass only reads t. */
+static int autotune(doci_batch *b)
+{
+    static const int cand[] = {512, 1024, 2048, 4096};
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    double best = 1e30;
+    int best_rows = 1024;
+    for (size_t c = 0; c < sizeof cand / sizeof cand[0]; ++c) {
+        b->tile_rows = cand[c];
+        int rc = rebuild(b);
+        if (rc != DOCI_OK) return rc;
+        float ms = 0;
+        for (int rep = 0; rep < 3; ++rep) {
+            HIPCHK(hipEventRecord(e0, b->stream));
+            for (int p = 0; p < 2; ++p) {
+                bgroup *g = &b->grp[p];
+                if (!g->n) continue;
+                DKCHK(dk_batch_sigma_lane(p, g->maxL, g->maxN, b->d_inst, b->d_state, g->d_tiles,
+                                          g->ntile, (uint32_t)b->tile_rows, b->stream));
+                DKCHK(dk_batch_stream(DK_BOP_ORTHO, p, 8, b->d_inst, b->d_state, g->d_tiles,
+                                      g->ntile, (uint32_t)b->tile_rows, g->d_part, b->stream));
+            }
+            HIPCHK(hipEventRecord(e1, b->stream));
+            HIPCHK(hipEventSynchronize(e1));
+            float t = 0;
+            HIPCHK(hipEventElapsedTime(&t, e0, e1));
+            if (rep > 0) ms += t; /* first repetition warms up */
+        }
+        if (ms < best) {
+            best = ms;
+            best_rows = cand[c];
+        }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    b->tile_rows = best_rows;
+    b->tuned = 1;
+    b->dirty = 1;
+    return DOCI_OK;
+}
+
+static int restart_fp64(doci_batch *b, brec *r, const dk_bstat *st)
+{
+    r->res.iterations_fp32 = st->its;
+    r->prec = 0;
+    DKCHK(dk_batch_start(&r->bi, b->d_state + r->slot, 1, 0, 0, b->o.tol, 1e-12, b->o.max_its,
+                         b->o.keep, b->o.max_vec, st->its, b->d_red, b->d_idx, b->stream));
+    b->dirty = 1;
+    return DOCI_OK;
+}
+
+static void finish(doci_batch *b, brec *r, const dk_bstat *st)
+{
+    r->res.status = st->status;
+    r->res.iterations = st->its;
+    if (r->prec) r->res.iterations_fp32 = st->its;
+    r->res.energy = st->energy;
+    r->res.residual = st->res;
+    r->res.t_done = doci_batch_clock(b);
+    r->state = 2;
+    b->slot_rec[r->slot] = -1;
+    r->slot = -1;
+    rec_free_main(b, r);
+    b->dirty = 1;
+}
+
+int doci_batch_step(doci_batch *b, int *running, int *waiting)
+{
+    int rc = admit_queued(b);
+    if (rc != DOCI_OK) return rc;
+    int nrun = 0;
+    for (int s = 0; s < b->cap; ++s) nrun += b->slot_rec[s] >= 0;
+    if (nrun) {
+        if (!b->tuned) {
+            rc = autotune(b);
+            if (rc != DOCI_OK) return rc;
+        }
+        if (b->dirty) {
+            rc = rebuild(b);
+            if (rc != DOCI_OK) return rc;
+        }
+        for (int p = 0; p < 2; ++p) {
+            bgroup *g = &b->grp[p];
+            if (!g->n) continue;
+            int mmax = 0;
+            for (int k = 0; k < g->n; ++k) {
+                const int m = b->h_stat[g->h_list[k]].m;
+                if (m > mmax) mmax = m;
+            }
+            rc = run_iteration(b, p, bucket_of(mmax + 1));
+            if (rc != DOCI_OK) return rc;
+        }
+        ++b->iterations;
+        HIPCHK(hipMemcpy2DAsync(b->h_stat, sizeof(dk_bstat), b->d_state, sizeof(dk_bstate),
+                                sizeof(dk_bstat), (size_t)b->cap, hipMemcpyDeviceToHost, b->stream));
+        HIPCHK(hipStreamSynchronize(b->stream));
+        for (int s = 0; s < b->cap; ++s) {
+            const int ri = b->slot_rec[s];
+            if (ri < 0) continue;
+            brec *r = &b->rec[ri];
+            const dk_bstat *st = &b->h_stat[s];
+            if (st->status == DK_B_ACTIVE) continue;
+            if (r->prec == 1 && st->status == DK_B_CONVERGED) {
+                rc = restart_fp64(b, r, st);
+                if (rc != DOCI_OK) return rc;
+            } else {
+                finish(b, r, st);
+            }
+        }
+        HIPCHK(hipStreamSynchronize(b->stream));
+    }
+    int nr = 0, nw = 0;
+    for (int i = 0; i < b->nrec; ++i) {
+        nr += b->rec[i].state == 1;
+        nw += b->rec[i].state == 0;
+    }
+    if (running) *running = nr;
+    if (waiting) *waiting = nw;
+    return DOCI_OK;
+}
+
+int doci_batch_run(doci_batch *b)
+{
+    for (;;) {
+        int nr = 0, nw = 0;
+        const int rc = doci_batch_step(b, &nr, &nw);
+        if (rc != DOCI_OK) return rc;
+        if (!nr && !nw) return DOCI_OK;
+        if (!nr && nw) return doci_fail(DOCI_E_NOMEM, "queued instances do not fit the memory budget");
+    }
+}
+
+int doci_batch_result_of(doci_batch *b, int id, doci_batch_result *res, double *x)
+{
+    if (id < 0 || id >= b->nrec) return doci_fail(DOCI_E_ARG, "no instance %d", id);
+    brec *r = &b->rec[id];
+    if (res) {
+        *res = r->res;
+        if (r->state == 1) {
+            const dk_bstat *st = &b->h_stat[r->slot];
+            res->iterations = st->its;
+            res->energy = st->energy;
+            res->residual = st->res;
+        }
+    }
+    if (x) {
+        if (r->state != 2 || !r->dx) return doci_fail(DOCI_E_ARG, "instance %d has no eigenvector", id);
+        HIPCHK(hipMemcpy(x, r->dx, sizeof(double) * r->n, hipMemcpyDeviceToHost));
+    }
+    return DOCI_OK;
+}
+
+int doci_batch_release(doci_batch *b, int id)
+{
+    if (id < 0 || id >= b->nrec) return doci_fail(DOCI_E_ARG, "no instance %d", id);
+    brec *r = &b->rec[id];
+    if (r->state != 2) return doci_fail(DOCI_E_ARG, "instance %d has not finished", id);
+    if (r->dx) hipFree(r->dx);
+    r->dx = NULL;
+    return DOCI_OK;
+}
+
+int doci_batch_stats(const doci_batch *b, int *tile_rows, int *iterations, int *graph_launches)
+{
+    if (tile_rows) *tile_rows = b->tile_rows;
+    if (iterations) *iterations = b->iterations;
+    if (graph_launches) *graph_launches = b->graph_launches;
+    return DOCI_OK;
+}

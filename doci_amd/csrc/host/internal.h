@@ -64,6 +64,10 @@ uint64_t dh_binom(unsigned k, unsigned n);
 
 /* operator.c */
 int dh_is_op(const void *p);
+/* K (zero diagonal), J, X (L x L), h' (L) and binomials ((L+1) x (N+2)) from
+ * raw integrals, transformed as src/doci.c:630-683. */
+int dh_build_tables(unsigned L, unsigned N, const double *one_raw, const double *two_raw,
+                    double *K, double *J, double *X, double *h, uint32_t *binom);
 
 /* dense.c: small dense eigensolvers */
 void dh_jacobi(int m, double *A, int lda, double *w, double *Z, int ldz);

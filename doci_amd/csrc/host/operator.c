@@ -61,6 +61,35 @@ void doci_op_free(doci_op *op)
     free(op);
 }
 
+int dh_build_tables(unsigned L, unsigned N, const double *one_raw, const double *two_raw,
+                    double *K, double *J, double *X, double *h, uint32_t *binom)
+{
+    const size_t l = L, l2 = l * l, l3 = l2 * l;
+    double *one = malloc(sizeof(double) * l2), *two = malloc(sizeof(double) * l2 * l2);
+    if (!one || !two) {
+        free(one);
+        free(two);
+        return DOCI_E_NOMEM;
+    }
+    memcpy(one, one_raw, sizeof(double) * l2);
+    memcpy(two, two_raw, sizeof(double) * l2 * l2);
+    dh_transform_integrals(one, two, L);
+    const int N2 = (int)N + 2;
+    for (size_t p = 0; p < l; ++p) {
+        h[p] = one[p * (l + 1)];
+        for (size_t q = 0; q < l; ++q) {
+            J[p * l + q] = two[p * (l + 1) + q * (l2 + l3)];          /* (pp|qq) */
+            X[p * l + q] = two[p * (1 + l3) + q * (l + l2)];          /* (pq|qp) */
+            K[p * l + q] = p == q ? 0.0 : two[p * (l2 + 1) + q * (l3 + l)]; /* (pq|pq) */
+        }
+    }
+    for (size_t x = 0; x <= l; ++x)
+        for (int j = 0; j < N2; ++j) binom[x * N2 + j] = (uint32_t)dh_binom((unsigned)j, (unsigned)x);
+    free(one);
+    free(two);
+    return DOCI_OK;
+}
+
 /* Takes ownership of nothing: one/two are transformed in a private copy. */
 int doci_op_from_integrals(unsigned L, unsigned N, double core, const double *one_raw,
                            const double *two_raw, doci_op **out)
@@ -71,24 +100,14 @@ int doci_op_from_integrals(unsigned L, unsigned N, double core, const double *on
     const uint64_t n = dh_binom(N, L);
     if (n >= (1ull << 31))
         return doci_fail(DOCI_E_DIM, "C(%u,%u) = %llu exceeds 2^31-1", L, N, (unsigned long long)n);
-    const size_t l = L, l2 = l * l, l3 = l2 * l;
+    const size_t l = L, l2 = l * l;
     doci_op *op = calloc(1, sizeof *op);
-    double *one = malloc(sizeof(double) * l2), *two = malloc(sizeof(double) * l2 * l2);
-    if (!op || !one || !two) {
-        free(op);
-        free(one);
-        free(two);
-        return doci_fail(DOCI_E_NOMEM, "host tables");
-    }
+    if (!op) return doci_fail(DOCI_E_NOMEM, "host tables");
     op->magic = DOCI_OP_MAGIC;
     op->L = L;
     op->N = N;
     op->n = (int64_t)n;
     op->core = core;
-    memcpy(one, one_raw, sizeof(double) * l2);
-    memcpy(two, two_raw, sizeof(double) * l2 * l2);
-    dh_transform_integrals(one, two, L);
-
     const int N2 = (int)N + 2;
     op->K = malloc(sizeof(double) * l2);
     op->J = malloc(sizeof(double) * l2);
@@ -96,23 +115,13 @@ int doci_op_from_integrals(unsigned L, unsigned N, double core, const double *on
     op->h = malloc(sizeof(double) * l);
     op->binom = malloc(sizeof(uint32_t) * (l + 1) * (size_t)N2);
     if (!op->K || !op->J || !op->X || !op->h || !op->binom) {
-        free(one);
-        free(two);
         doci_op_free(op);
         return doci_fail(DOCI_E_NOMEM, "host tables");
     }
-    for (size_t p = 0; p < l; ++p) {
-        op->h[p] = one[p * (l + 1)];
-        for (size_t q = 0; q < l; ++q) {
-            op->J[p * l + q] = two[p * (l + 1) + q * (l2 + l3)];      /* (pp|qq) */
-            op->X[p * l + q] = two[p * (1 + l3) + q * (l + l2)];      /* (pq|qp) */
-            op->K[p * l + q] = p == q ? 0.0 : two[p * (l2 + 1) + q * (l3 + l)]; /* (pq|pq) */
-        }
+    if (dh_build_tables(L, N, one_raw, two_raw, op->K, op->J, op->X, op->h, op->binom) != DOCI_OK) {
+        doci_op_free(op);
+        return doci_fail(DOCI_E_NOMEM, "host tables");
     }
-    for (size_t x = 0; x <= l; ++x)
-        for (int j = 0; j < N2; ++j) op->binom[x * N2 + j] = (uint32_t)dh_binom((unsigned)j, (unsigned)x);
-    free(one);
-    free(two);
 
     /* device side */
     int rc = DOCI_OK;

@@ -1,0 +1,667 @@
+// batch.hip -- many independent DOCI Davidson solves in lock step.
+//
+// The Davidson iteration of src/davidson.c:72-99 is split into streaming
+// steps (one workgroup per tile = a row block of one instance; all
+// instances in one launch) and small steps (one 64-lane workgroup per
+// instance) that reduce the tile partials in a fixed order and advance that
+// instance's state:
+//
+//   small A   residual norm, loop test `res > tol && its++ < max_its`,
+//             Gram-Schmidt coefficients of t
+//   ORTHO     t -= sum g_j V_j  (+ norms and overlaps for the check)
+//   small B   loss of orthogonality -> second pass, or scale of V_m
+//   ORTHO2    second classical Gram-Schmidt pass (flagged instances only)
+//   small C   scale of V_m after the second pass
+//   sigma     W_m = (H - shift) V_m
+//   SCOL      new column V_j^T W_m of the projected matrix
+//   small D   bordered update of the projected eigenproblem (cyclic Jacobi,
+//             parallel ordering), lowest Ritz pair; deflation plan when the
+//             basis reaches max_vec (src/davidson.c:83-89)
+//   DEFLATE   V'_k = sum_j Y_jk V_j, W'_k likewise (flagged instances only)
+//   small E   basis = the kept Ritz vectors
+//   RESID     x = V y, r = W y - theta x, t = r / |D - E| with the 1e-12
+//             floor (src/davidson.c:214-227), x written in FP64
+//
+// Every launch reads each instance's state from device memory, so the same
+// launch sequence serves every iteration and can be captured in a HIP graph.
+// Vectors are stored unnormalised with a scale per basis vector (V_j =
+// s_j * stored_j), which saves a normalisation pass.
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "../dk.h"
+#include "addressing.hpp"
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kWaves = kBlock / 64;
+constexpr int kSmall = 64;
+constexpr int LDY = DK_BMAX_VEC;
+
+template <typename T>
+__device__ __forceinline__ T *slot_ptr(const dk_binst &in, int slot)
+{
+    return (T *)(in.slots + (uint64_t)slot * in.slot_stride);
+}
+
+template <int NV>
+__device__ __forceinline__ void tile_partials(double (&acc)[NV], int nv, double *out)
+{
+    __shared__ double sh[NV][kWaves];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NV; ++k) {
+        if (k < nv) {
+            double v = acc[k];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+            if (lane == 0) sh[k][wave] = v;
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < nv; k += kBlock) {
+        double s = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) s += sh[k][w];
+        out[k] = s;
+    }
+}
+
+// ---- streaming steps -------------------------------------------------------
+template <typename T, int M, int OP>
+__global__ __launch_bounds__(kBlock) void k_bstream(const dk_binst *__restrict__ insts,
+                                                    dk_bstate *__restrict__ states,
+                                                    const uint32_t *__restrict__ tiles,
+                                                    uint32_t tile_rows, double *__restrict__ part)
+{
+    const uint32_t ii = tiles[2 * blockIdx.x], ti = tiles[2 * blockIdx.x + 1];
+    const dk_binst in = insts[ii];
+    const dk_bstate &st = states[ii];
+    if (st.st.status != DK_B_ACTIVE) return;
+    const int flags = st.st.flags;
+    if (OP == DK_BOP_ORTHO2 && !(flags & DK_BF_ORTHO2)) return;
+    if (OP == DK_BOP_DEFLATE && !(flags & DK_BF_DEFLATE)) return;
+    const int m = st.st.m;
+    const uint32_t r0 = ti * tile_rows;
+    const uint32_t r1 = min(in.n, r0 + tile_rows);
+    double *out = part + (size_t)blockIdx.x * DK_BPV;
+
+    const T *V[M];
+#pragma unroll
+    for (int j = 0; j < M; ++j) V[j] = j < m ? slot_ptr<T>(in, st.vslot[j]) : nullptr;
+
+    if constexpr (OP == DK_BOP_ORTHO || OP == DK_BOP_ORTHO2) {
+        T *t = slot_ptr<T>(in, st.tslot);
+        double g[M];
+#pragma unroll
+        for (int j = 0; j < M; ++j) g[j] = j < m ? st.g[j] : 0.0;
+        double acc[M + 1];
+#pragma unroll
+        for (int k = 0; k <= M; ++k) acc[k] = 0;
+        for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
+            double v[M];
+            double tv = (double)t[i];
+#pragma unroll
+            for (int j = 0; j < M; ++j)
+                if (j < m) {
+                    v[j] = (double)V[j][i];
+                    tv -= g[j] * v[j];
+                }
+            const T tq = (T)tv;
+            if (m > 0) t[i] = tq;
+            const double td = (double)tq;
+            acc[0] += td * td;
+#pragma unroll
+            for (int j = 0; j < M; ++j)
+                if (j < m) acc[1 + j] += v[j] * td;
+        }
+        tile_partials<M + 1>(acc, m + 1, out);
+    } else if constexpr (OP == DK_BOP_SCOL) {
+        // column m: V_0..V_m (V_m is t itself) against W_m, the sigma just computed
+        const T *Vp[M + 1];
+#pragma unroll
+        for (int j = 0; j <= M; ++j) Vp[j] = j <= m ? slot_ptr<T>(in, st.vslot[j]) : nullptr;
+        const T *Wm = slot_ptr<T>(in, st.wslot[m]);
+        double acc[M + 1];
+#pragma unroll
+        for (int k = 0; k <= M; ++k) acc[k] = 0;
+        for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
+            const double w = (double)Wm[i];
+#pragma unroll
+            for (int j = 0; j <= M; ++j)
+                if (j <= m) acc[j] += (double)Vp[j][i] * w;
+        }
+        tile_partials<M + 1>(acc, m + 1, out);
+    } else if constexpr (OP == DK_BOP_DEFLATE) {
+        const int keep = st.keep;
+        const T *W[M];
+#pragma unroll
+        for (int j = 0; j < M; ++j) W[j] = j < m ? slot_ptr<T>(in, st.wslot[j]) : nullptr;
+        for (int k = 0; k < keep; ++k) {
+            double c[M];
+#pragma unroll
+            for (int j = 0; j < M; ++j) c[j] = j < m ? st.defl[k][j] : 0.0;
+            T *dv = slot_ptr<T>(in, st.dv[k]);
+            T *dw = slot_ptr<T>(in, st.dw[k]);
+            for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
+                double a = 0, b = 0;
+#pragma unroll
+                for (int j = 0; j < M; ++j)
+                    if (j < m) {
+                        a += c[j] * (double)V[j][i];
+                        b += c[j] * (double)W[j][i];
+                    }
+                dv[i] = (T)a;
+                dw[i] = (T)b;
+            }
+        }
+    } else { // DK_BOP_RESID
+        T *t = slot_ptr<T>(in, st.tslot);
+        const double E = st.theta + st.shift, theta = st.theta;
+        double yc[M];
+        const T *W[M];
+#pragma unroll
+        for (int j = 0; j < M; ++j) {
+            yc[j] = j < m ? st.y[j] : 0.0;
+            W[j] = j < m ? slot_ptr<T>(in, st.wslot[j]) : nullptr;
+        }
+        double acc[M + 2];
+#pragma unroll
+        for (int k = 0; k < M + 2; ++k) acc[k] = 0;
+        for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
+            double v[M];
+            double x = 0, w = 0;
+#pragma unroll
+            for (int j = 0; j < M; ++j)
+                if (j < m) {
+                    v[j] = (double)V[j][i];
+                    x += yc[j] * v[j];
+                    w += yc[j] * (double)W[j][i];
+                }
+            const double r = w - theta * x;
+            const double d = fabs(in.D[i] - E);
+            const double tv = d > 1e-12 ? r / d : r / 1e-12;
+            const T tq = (T)tv;
+            t[i] = tq;
+            in.x[i] = x;
+            const double td = (double)tq;
+            acc[0] += r * r;
+            acc[1] += td * td;
+#pragma unroll
+            for (int j = 0; j < M; ++j)
+                if (j < m) acc[2 + j] += v[j] * td;
+        }
+        tile_partials<M + 2>(acc, m + 2, out);
+    }
+}
+
+// ---- batched lane sigma ----------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bsigma_lane(const dk_binst *__restrict__ insts,
+                                                        const dk_bstate *__restrict__ states,
+                                                        const uint32_t *__restrict__ tiles,
+                                                        uint32_t tile_rows)
+{
+    extern __shared__ double lds[];
+    const uint32_t ii = tiles[2 * blockIdx.x], ti = tiles[2 * blockIdx.x + 1];
+    const dk_binst in = insts[ii];
+    const dk_bstate &st = states[ii];
+    if (st.st.status != DK_B_ACTIVE || !in.lane_sigma) return;
+    const int L = (int)in.L, N = (int)in.N, N2 = N + 2;
+    double *K = lds;
+    uint32_t *bt = (uint32_t *)(K + L * L);
+    for (int i = threadIdx.x; i < L * L; i += kBlock) K[i] = in.K[i];
+    for (int i = threadIdx.x; i < (L + 1) * N2; i += kBlock) bt[i] = in.binom[i];
+    __syncthreads();
+    const T *c = slot_ptr<T>(in, st.sin);
+    T *sigma = slot_ptr<T>(in, st.sout);
+    const double shift = st.shift;
+    const dk::Binom C{bt, N2};
+    const uint32_t r0 = ti * tile_rows;
+    const uint32_t r1 = min(in.n, r0 + tile_rows);
+    for (uint32_t I = r0 + threadIdx.x; I < r1; I += kBlock) {
+        const uint64_t msk = dk::unrank(I, L, N, C);
+        double acc = (in.D[I] - shift) * (double)c[I];
+        dk::for_each_hop(I, msk, L, N, C, [&](int o, int tt, uint32_t Jr) {
+            acc += K[o * L + tt] * (double)c[Jr];
+        });
+        sigma[I] = (T)acc;
+    }
+}
+
+// ---- small steps -------------------------------------------------------
+// Sum value k over the instance's tiles, k < nv, in a fixed order.
+__device__ void reduce_tiles(const dk_binst &in, const double *__restrict__ part, int nv,
+                             double *res /* LDS, DK_BPV */)
+{
+    __shared__ double sh[DK_BPV][kSmall + 1];
+    double acc[DK_BPV];
+#pragma unroll
+    for (int k = 0; k < DK_BPV; ++k) acc[k] = 0;
+    for (uint32_t t = threadIdx.x; t < in.ntiles; t += kSmall) {
+        const double *p = part + (size_t)(in.tile0 + t) * DK_BPV;
+#pragma unroll
+        for (int k = 0; k < DK_BPV; ++k)
+            if (k < nv) acc[k] += p[k];
+    }
+#pragma unroll
+    for (int k = 0; k < DK_BPV; ++k) sh[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    if (threadIdx.x < nv) {
+        double s = 0;
+        for (int q = 0; q < kSmall; ++q) s += sh[threadIdx.x][q];
+        res[threadIdx.x] = s;
+    }
+    __syncthreads();
+}
+
+// lowest free slot of the pool [lo, lo + count); bit k of `used` = slot lo + k
+__device__ int take_slot(uint64_t &used, int lo, int count)
+{
+    for (int k = 0; k < count; ++k)
+        if (!((used >> k) & 1)) {
+            used |= 1ull << k;
+            return lo + k;
+        }
+    return -1;
+}
+
+// Cyclic Jacobi on the leading n x n block of A (row-major, stride LDY),
+// rotations accumulated into Y; pairs of a sweep run in parallel in the
+// round-robin (tournament) order, so the disjoint rotations of one step
+// commute.  Stops when the off-diagonal mass is below 1e-32 of the total.
+__device__ void jacobi_par(double *A, double *Y, int n)
+{
+    __shared__ double cs[DK_BMAX_VEC / 2][2];
+    __shared__ int pr[DK_BMAX_VEC / 2][2];
+    __shared__ double red[2][kSmall];
+    __shared__ int done;
+    const int nn = n + (n & 1);
+    const int npair = nn / 2;
+    for (int sweep = 0; sweep < 40; ++sweep) {
+        double off = 0, all = 0;
+        for (int e = threadIdx.x; e < n * n; e += kSmall) {
+            const int i = e / n, j = e - i * n;
+            const double a = A[i * LDY + j];
+            all += a * a;
+            if (i != j) off += a * a;
+        }
+        red[0][threadIdx.x] = off;
+        red[1][threadIdx.x] = all;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double o = 0, a = 0;
+            for (int q = 0; q < kSmall; ++q) {
+                o += red[0][q];
+                a += red[1][q];
+            }
+            done = (o == 0 || o <= 1e-32 * a);
+        }
+        __syncthreads();
+        if (done) break;
+        for (int step = 0; step < nn - 1; ++step) {
+            if (threadIdx.x < npair) {
+                // tournament pairing: index nn-1 fixed, the others rotate
+                const int k = threadIdx.x;
+                int p, q;
+                if (k == 0) {
+                    p = nn - 1;
+                    q = step;
+                } else {
+                    p = (step + k) % (nn - 1);
+                    q = (step - k + (nn - 1)) % (nn - 1);
+                }
+                if (p > q) {
+                    const int z = p;
+                    p = q;
+                    q = z;
+                }
+                double c = 1, s = 0;
+                if (q < n) {
+                    const double apq = A[p * LDY + q];
+                    if (apq != 0) {
+                        const double tau = (A[q * LDY + q] - A[p * LDY + p]) / (2 * apq);
+                        const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1 + tau * tau));
+                        c = 1 / sqrt(1 + t * t);
+                        s = t * c;
+                    }
+                }
+                pr[k][0] = p;
+                pr[k][1] = q;
+                cs[k][0] = c;
+                cs[k][1] = s;
+            }
+            __syncthreads();
+            // rows p, q of A (A <- J^T A)
+            for (int e = threadIdx.x; e < npair * n; e += kSmall) {
+                const int k = e / n, col = e - k * n;
+                const int p = pr[k][0], q = pr[k][1];
+                if (q >= n || cs[k][1] == 0) continue;
+                const double c = cs[k][0], s = cs[k][1];
+                const double x = A[p * LDY + col], y = A[q * LDY + col];
+                A[p * LDY + col] = c * x - s * y;
+                A[q * LDY + col] = s * x + c * y;
+            }
+            __syncthreads();
+            // columns p, q of A and Y (A <- A J, Y <- Y J)
+            for (int e = threadIdx.x; e < npair * n; e += kSmall) {
+                const int k = e / n, row = e - k * n;
+                const int p = pr[k][0], q = pr[k][1];
+                if (q >= n || cs[k][1] == 0) continue;
+                const double c = cs[k][0], s = cs[k][1];
+                double x = A[row * LDY + p], y = A[row * LDY + q];
+                A[row * LDY + p] = c * x - s * y;
+                A[row * LDY + q] = s * x + c * y;
+                x = Y[row * LDY + p];
+                y = Y[row * LDY + q];
+                Y[row * LDY + p] = c * x - s * y;
+                Y[row * LDY + q] = s * x + c * y;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__global__ __launch_bounds__(kSmall) void k_bsmall(int phase, const uint32_t *__restrict__ ilist,
+                                                   const dk_binst *__restrict__ insts,
+                                                   dk_bstate *__restrict__ states,
+                                                   const double *__restrict__ part)
+{
+    __shared__ double red[DK_BPV];
+    __shared__ double A[DK_BMAX_VEC * LDY], Y[DK_BMAX_VEC * LDY];
+    __shared__ int ctl;
+    const uint32_t ii = ilist[blockIdx.x];
+    const dk_binst in = insts[ii];
+    dk_bstate &st = states[ii];
+    if (st.st.status != DK_B_ACTIVE) return;
+    const int m = st.st.m;
+    const int flags = st.st.flags;
+
+    if (phase == DK_BPH_A) {
+        if (!(flags & DK_BF_FIRST)) reduce_tiles(in, part, m + 2, red);
+        if (threadIdx.x == 0) {
+            const bool first = flags & DK_BF_FIRST;
+            const double res = first ? st.tol * 10 : sqrt(red[0]); // src/davidson.c:45
+            st.st.res = res;
+            // src/davidson.c:72: res > tol && its++ < max_its
+            bool cont = res > st.tol;
+            if (cont) cont = st.st.its++ < st.max_its;
+            ctl = 0;
+            if (!cont)
+                st.st.status = res > st.tol ? DK_B_MAXITS : DK_B_CONVERGED;
+            else if (!first && (!(red[1] > 0) || !isfinite(red[0]) || !isfinite(red[1])))
+                st.st.status = DK_B_BREAKDOWN;
+            else
+                ctl = 1;
+            st.st.flags = flags & ~DK_BF_FIRST;
+        }
+        __syncthreads();
+        if (ctl && !(flags & DK_BF_FIRST))
+            for (int j = threadIdx.x; j < m; j += kSmall) st.g[j] = st.s[j] * st.s[j] * red[2 + j];
+        return;
+    }
+    if (phase == DK_BPH_B || phase == DK_BPH_C) {
+        if (phase == DK_BPH_C && !(flags & DK_BF_ORTHO2)) return;
+        reduce_tiles(in, part, m + 1, red);
+        if (threadIdx.x == 0) {
+            const double tn = sqrt(red[0]);
+            double loss = 0;
+            for (int j = 0; j < m; ++j) loss = fmax(loss, fabs(st.s[j] * red[1 + j]));
+            ctl = 0;
+            if (!(tn > 0) || !isfinite(tn)) {
+                st.st.status = DK_B_BREAKDOWN;
+            } else if (phase == DK_BPH_B && m > 0 && loss > st.otol * tn) {
+                st.st.flags = flags | DK_BF_ORTHO2;
+                ctl = 1; // second pass coefficients below
+            } else {
+                // t becomes V_m with scale 1/|t|; W_m goes to a free W slot
+                st.s[m] = 1 / tn;
+                st.vslot[m] = st.tslot;
+                const int w = take_slot(st.wused, st.P, st.P);
+                st.wslot[m] = w;
+                st.sin = st.tslot;
+                st.sout = w;
+                st.st.flags = flags & ~DK_BF_ORTHO2;
+                if (w < 0) st.st.status = DK_B_BREAKDOWN;
+            }
+        }
+        __syncthreads();
+        if (ctl)
+            for (int j = threadIdx.x; j < m; j += kSmall) st.g[j] = st.s[j] * st.s[j] * red[1 + j];
+        return;
+    }
+    if (phase == DK_BPH_D) {
+        reduce_tiles(in, part, m + 1, red);
+        const int mn = m + 1;
+        // column m of the projected matrix in true (normalised) vectors
+        __shared__ double col[DK_BMAX_VEC];
+        for (int j = threadIdx.x; j <= m; j += kSmall) col[j] = st.s[j] * st.s[m] * red[j];
+        for (int e = threadIdx.x; e < m * m; e += kSmall) {
+            const int i = e / m, j = e - i * m;
+            A[i * LDY + j] = st.A[i * LDY + j];
+            Y[i * LDY + j] = st.Y[i * LDY + j];
+        }
+        __syncthreads();
+        // border the rotated matrix: b = Y^T col, A' = [A b; b^T col_m]
+        for (int k = threadIdx.x; k < m; k += kSmall) {
+            double b = 0;
+            for (int j = 0; j < m; ++j) b += Y[j * LDY + k] * col[j];
+            A[k * LDY + m] = b;
+            A[m * LDY + k] = b;
+            Y[k * LDY + m] = 0;
+            Y[m * LDY + k] = 0;
+        }
+        if (threadIdx.x == 0) {
+            A[m * LDY + m] = col[m];
+            Y[m * LDY + m] = 1;
+        }
+        __syncthreads();
+        jacobi_par(A, Y, mn);
+        __shared__ int order[DK_BMAX_VEC];
+        if (threadIdx.x == 0) {
+            // ascending eigenvalues (selection order; ties keep the lower index)
+            int used = 0;
+            for (int r = 0; r < mn; ++r) {
+                int best = -1;
+                for (int k = 0; k < mn; ++k)
+                    if (!((used >> k) & 1) && (best < 0 || A[k * LDY + k] < A[best * LDY + best]))
+                        best = k;
+                order[r] = best;
+                used |= 1 << best;
+            }
+            st.theta = A[order[0] * LDY + order[0]];
+            st.st.energy = st.theta + st.shift;
+            st.st.m = mn;
+            ctl = mn == st.max_vec;
+            if (ctl) {
+                for (int k = 0; k < st.keep; ++k) {
+                    st.dv[k] = take_slot(st.vused, 0, st.P);
+                    st.dw[k] = take_slot(st.wused, st.P, st.P);
+                    if (st.dv[k] < 0 || st.dw[k] < 0) st.st.status = DK_B_BREAKDOWN;
+                }
+                st.st.flags |= DK_BF_DEFLATE;
+            } else {
+                st.tslot = take_slot(st.vused, 0, st.P);
+                if (st.tslot < 0) st.st.status = DK_B_BREAKDOWN;
+            }
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < mn * mn; e += kSmall) {
+            const int i = e / mn, j = e - i * mn;
+            st.A[i * LDY + j] = A[i * LDY + j];
+            st.Y[i * LDY + j] = Y[i * LDY + j];
+        }
+        if (ctl) {
+            for (int e = threadIdx.x; e < st.keep * mn; e += kSmall) {
+                const int k = e / mn, j = e - k * mn;
+                st.defl[k][j] = Y[j * LDY + order[k]] * st.s[j];
+            }
+            // the kept Ritz values, for small E
+            for (int k = threadIdx.x; k < st.keep; k += kSmall) st.g[k] = A[order[k] * LDY + order[k]];
+        } else {
+            for (int j = threadIdx.x; j < mn; j += kSmall) st.y[j] = Y[j * LDY + order[0]] * st.s[j];
+        }
+        return;
+    }
+    if (phase == DK_BPH_E) {
+        if (!(flags & DK_BF_DEFLATE)) return;
+        if (threadIdx.x == 0) {
+            const int keep = st.keep;
+            for (int j = 0; j < m; ++j) {
+                st.vused &= ~(1ull << st.vslot[j]);
+                st.wused &= ~(1ull << (st.wslot[j] - st.P));
+            }
+            for (int k = 0; k < keep; ++k) {
+                st.vslot[k] = st.dv[k];
+                st.wslot[k] = st.dw[k];
+                st.s[k] = 1;
+                st.y[k] = k == 0 ? 1.0 : 0.0;
+            }
+            for (int i = 0; i < keep; ++i)
+                for (int j = 0; j < keep; ++j) {
+                    st.A[i * LDY + j] = i == j ? st.g[i] : 0.0;
+                    st.Y[i * LDY + j] = i == j ? 1.0 : 0.0;
+                }
+            st.theta = st.g[0];
+            st.st.energy = st.theta + st.shift;
+            st.st.m = keep;
+            st.st.flags = flags & ~DK_BF_DEFLATE;
+            st.tslot = take_slot(st.vused, 0, st.P);
+            if (st.tslot < 0) st.st.status = DK_B_BREAKDOWN;
+        }
+        return;
+    }
+}
+
+// ---- admission -------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bguess(T *__restrict__ t, const double *__restrict__ x,
+                                                   uint32_t n, const int64_t *__restrict__ idx)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    if (x)
+        t[i] = (T)x[i];
+    else
+        t[i] = (T)((int64_t)i == *idx ? 1.0 : 0.0);
+}
+
+__global__ void k_bstart(dk_bstate *st, const double *__restrict__ D, const int64_t *__restrict__ idx,
+                         int shift_min_d, double tol, double otol, int max_its, int keep,
+                         int max_vec, int its0)
+{
+    if (threadIdx.x != 0) return;
+    st->st.status = DK_B_ACTIVE;
+    st->st.m = 0;
+    st->st.its = its0;
+    st->st.flags = DK_BF_FIRST;
+    st->st.res = tol * 10;
+    st->max_its = max_its;
+    st->keep = keep;
+    st->max_vec = max_vec;
+    st->P = max_vec + keep;
+    st->tslot = 0;
+    st->sin = 0;
+    st->sout = max_vec + keep; // first W slot: a sigma probe before the first iteration writes there
+    st->vused = 1;
+    st->wused = 0;
+    st->tol = tol;
+    st->otol = otol;
+    st->shift = shift_min_d ? D[*idx] : 0.0;
+    st->theta = 0;
+    st->st.energy = st->shift;
+}
+
+template <typename T, int M>
+int stream_launch(int op, const dk_binst *inst, dk_bstate *state, const uint32_t *tiles,
+                  uint32_t ntile, uint32_t rows, double *part, hipStream_t s)
+{
+    const dim3 g(ntile), b(kBlock);
+    switch (op) {
+    case DK_BOP_ORTHO:
+        hipLaunchKernelGGL((k_bstream<T, M, DK_BOP_ORTHO>), g, b, 0, s, inst, state, tiles, rows, part);
+        break;
+    case DK_BOP_ORTHO2:
+        hipLaunchKernelGGL((k_bstream<T, M, DK_BOP_ORTHO2>), g, b, 0, s, inst, state, tiles, rows, part);
+        break;
+    case DK_BOP_SCOL:
+        hipLaunchKernelGGL((k_bstream<T, M, DK_BOP_SCOL>), g, b, 0, s, inst, state, tiles, rows, part);
+        break;
+    case DK_BOP_DEFLATE:
+        hipLaunchKernelGGL((k_bstream<T, M, DK_BOP_DEFLATE>), g, b, 0, s, inst, state, tiles, rows, part);
+        break;
+    case DK_BOP_RESID:
+        hipLaunchKernelGGL((k_bstream<T, M, DK_BOP_RESID>), g, b, 0, s, inst, state, tiles, rows, part);
+        break;
+    default:
+        return (int)hipErrorInvalidValue;
+    }
+    return (int)hipGetLastError();
+}
+
+template <typename T>
+int stream_bucket(int op, int M, const dk_binst *inst, dk_bstate *state, const uint32_t *tiles,
+                  uint32_t ntile, uint32_t rows, double *part, hipStream_t s)
+{
+    if (M <= 8) return stream_launch<T, 8>(op, inst, state, tiles, ntile, rows, part, s);
+    if (M <= 16) return stream_launch<T, 16>(op, inst, state, tiles, ntile, rows, part, s);
+    if (M <= DK_BMAX_VEC) return stream_launch<T, DK_BMAX_VEC>(op, inst, state, tiles, ntile, rows, part, s);
+    return (int)hipErrorInvalidValue;
+}
+
+} // namespace
+
+extern "C" int dk_batch_stream(int op, int prec, int M, const dk_binst *inst, dk_bstate *state,
+                               const uint32_t *tiles, uint32_t ntile, uint32_t tile_rows,
+                               double *part, hipStream_t s)
+{
+    if (ntile == 0) return 0;
+    return prec ? stream_bucket<float>(op, M, inst, state, tiles, ntile, tile_rows, part, s)
+                : stream_bucket<double>(op, M, inst, state, tiles, ntile, tile_rows, part, s);
+}
+
+extern "C" int dk_batch_small(int phase, const uint32_t *ilist, uint32_t ninst,
+                              const dk_binst *inst, dk_bstate *state, const double *part,
+                              hipStream_t s)
+{
+    if (ninst == 0) return 0;
+    hipLaunchKernelGGL(k_bsmall, dim3(ninst), dim3(kSmall), 0, s, phase, ilist, inst, state, part);
+    return (int)hipGetLastError();
+}
+
+extern "C" int dk_batch_sigma_lane(int prec, uint32_t maxL, uint32_t maxN, const dk_binst *inst,
+                                   dk_bstate *state, const uint32_t *tiles, uint32_t ntile,
+                                   uint32_t tile_rows, hipStream_t s)
+{
+    if (ntile == 0) return 0;
+    const size_t lds = sizeof(double) * maxL * maxL + sizeof(uint32_t) * (maxL + 1) * (maxN + 2);
+    if (prec)
+        hipLaunchKernelGGL(k_bsigma_lane<float>, dim3(ntile), dim3(kBlock), lds, s, inst, state,
+                           tiles, tile_rows);
+    else
+        hipLaunchKernelGGL(k_bsigma_lane<double>, dim3(ntile), dim3(kBlock), lds, s, inst, state,
+                           tiles, tile_rows);
+    return (int)hipGetLastError();
+}
+
+extern "C" int dk_batch_start(const dk_binst *in, dk_bstate *state_i, int from_x, int prec,
+                              int shift_min_d, double tol, double otol, int max_its, int keep,
+                              int max_vec, int its0, double *red, int64_t *idx, hipStream_t s)
+{
+    int e = dk_argmin(in->D, (int64_t)in->n, red, idx, s);
+    if (e) return e;
+    const dim3 g((in->n + kBlock - 1) / kBlock), b(kBlock);
+    void *t = in->slots; // slot 0 holds t at admission
+    if (prec)
+        hipLaunchKernelGGL(k_bguess<float>, g, b, 0, s, (float *)t, from_x ? in->x : nullptr, in->n,
+                           idx);
+    else
+        hipLaunchKernelGGL(k_bguess<double>, g, b, 0, s, (double *)t, from_x ? in->x : nullptr,
+                           in->n, idx);
+    e = (int)hipGetLastError();
+    if (e) return e;
+    hipLaunchKernelGGL(k_bstart, dim3(1), dim3(64), 0, s, state_i, in->D, idx, shift_min_d, tol,
+                       otol, max_its, keep, max_vec, its0);
+    return (int)hipGetLastError();
+}

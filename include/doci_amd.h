@@ -221,6 +221,63 @@ typedef struct {
 /* Reads the reference's key = value input file with its defaults. */
 int doci_read_inputfile(const char *path, doci_input *in);
 
+/* ---- many independent solves on one GPU ----------------------------------
+ * No counterpart in the reference (one solve per process there).  A batch
+ * holds DOCI instances of any (L, N) that iterate Davidson in lock step:
+ * every iteration is one fixed sequence of launches covering all running
+ * instances (captured in a HIP graph), each instance advancing its own state
+ * on the device.  Instances can be added while others run (serving): they
+ * are admitted at the next iteration boundary, as long as the vector memory
+ * of the running instances stays within `mem_budget`.  Results per instance
+ * equal those of an independent solve within the solver tolerances. */
+typedef struct {
+    double tol;            /* 1e-8 */
+    int max_its;           /* 200 */
+    int keep;              /* 2 (at most 4) */
+    int max_vec;           /* 30 (at most 30) */
+    double switch_tol;     /* mixed instances: FP32 phase until this residual (1e-5) */
+    int use_graphs;        /* capture each iteration's launches in a HIP graph (1) */
+    int tile_rows;         /* rows per workgroup of the batched launches; 0 = autotune */
+    size_t mem_budget;     /* bytes of device memory for running instances (0 = 90% free) */
+    int max_instances;     /* capacity (1024) */
+} doci_batch_opts;
+
+typedef struct doci_batch doci_batch;
+
+typedef struct {
+    int status;            /* 0 waiting, 1 running, 2 converged, 3 max_its, 4 breakdown */
+    int iterations;        /* total (both phases) */
+    int iterations_fp32;
+    double energy;
+    double residual;
+    double t_submit, t_start, t_done; /* seconds on the batch's clock */
+} doci_batch_result;
+
+void doci_batch_defaults(doci_batch_opts *o);
+int doci_batch_create(const doci_batch_opts *o, doci_batch **b);
+void doci_batch_free(doci_batch *b);
+/* Device bytes an instance of (L, N) holds while it runs. */
+size_t doci_batch_footprint(const doci_batch_opts *o, unsigned L, unsigned N);
+/* Queue an instance (raw integrals as in doci_op_from_integrals; copied).
+ * precision 0 = FP64, 1 = mixed.  *id identifies it in later calls. */
+int doci_batch_add(doci_batch *b, unsigned L, unsigned N, double core, const double *one,
+                   const double *two, int precision, int *id);
+/* Admit what fits, run one Davidson iteration of every running instance,
+ * retire finished ones.  *running / *waiting (may be NULL) receive the
+ * counts afterwards. */
+int doci_batch_step(doci_batch *b, int *running, int *waiting);
+/* Step until every queued instance has finished. */
+int doci_batch_run(doci_batch *b);
+/* Status of an instance; x (may be NULL) receives its unit eigenvector
+ * (C(L,N) doubles) once finished. */
+int doci_batch_result_of(doci_batch *b, int id, doci_batch_result *r, double *x);
+/* Free a finished instance's device memory (its result stays readable). */
+int doci_batch_release(doci_batch *b, int id);
+/* Seconds on the batch's clock (the clock of t_submit / t_start / t_done). */
+double doci_batch_clock(const doci_batch *b);
+/* Rows per workgroup in use (after autotuning), iterations run, graph launches. */
+int doci_batch_stats(const doci_batch *b, int *tile_rows, int *iterations, int *graph_launches);
+
 #ifdef __cplusplus
 }
 #endif
