@@ -65,6 +65,90 @@ def measured_traffic(op, prec: str):
         return round(json.load(f)["hbm_bytes_per_sigma"][prec] / 1e9, 2)
 
 
+def batched_leg(da, count: int) -> dict:
+    """The batched-solve set (doci_amd.workloads.batched_instances): all
+    instances queued at once and run to convergence in one batch, FP64 and
+    mixed; baseline = the same instances as plain FP64 single solves one
+    after the other.  Integrals are generated beforehand (not timed)."""
+    import numpy as np
+    from doci_amd.workloads import batched_instances
+    inst = batched_instances(count)
+    ints = [da.gen_integrals(L, s) for L, N, s in inst]
+    out = {"instances": count, "dims": [min(da.binom(N, L) for L, N, _ in inst),
+                                        max(da.binom(N, L) for L, N, _ in inst)]}
+    energies = {}
+    for prec in ("fp64", "mixed"):
+        t = time.perf_counter()
+        with da.Batch() as b:
+            ids = [b.add(L, N, *ints[k], precision=prec) for k, (L, N, _) in enumerate(inst)]
+            b.run()
+            t = time.perf_counter() - t
+            res = [b.result(i)[0] for i in ids]
+            st = b.stats()
+        energies[prec] = np.array([r.energy for r in res])
+        out[prec] = {"seconds": round(t, 3), "solves_per_s": round(count / t, 1),
+                     "lockstep_iterations": st["iterations"], "tile_rows": st["tile_rows"],
+                     "graph_launches": st["graph_launches"],
+                     "all_converged": all(r.status == 2 for r in res),
+                     "max_iterations": max(r.iterations for r in res)}
+        log(f"batched {prec}: {count} solves in {t:.3f} s ({count / t:.1f}/s), "
+            f"{st['iterations']} lock-step iterations, tile rows {st['tile_rows']}")
+    t = time.perf_counter()
+    e1 = []
+    for k, (L, N, _) in enumerate(inst):
+        with da.Operator.from_integrals(L, N, *ints[k]) as op:
+            e1.append(op.solve("D").energy)
+    t1 = time.perf_counter() - t
+    e1 = np.array(e1)
+    out["one_at_a_time_fp64"] = {"seconds": round(t1, 3), "solves_per_s": round(count / t1, 1)}
+    out["speedup_fp64"] = round(t1 / out["fp64"]["seconds"], 2)
+    out["speedup_mixed"] = round(t1 / out["mixed"]["seconds"], 2)
+    out["max_abs_dE_vs_single"] = {p: float(np.abs(energies[p] - e1).max()) for p in energies}
+    log(f"one-at-a-time fp64: {t1:.3f} s; max |dE| {out['max_abs_dE_vs_single']}")
+    return out
+
+
+def serving_leg(da, count: int, rates, precision: str) -> dict:
+    """The serving stream (doci_amd.workloads.serving_requests) at Poisson
+    rates lam = r * mu0, mu0 = one-at-a-time plain FP64 throughput measured
+    on a sample of the same requests; latencies are arrival -> converged."""
+    import numpy as np
+    from doci_amd import serve
+    from doci_amd.workloads import poisson_arrivals, serving_requests
+    reqs = serving_requests(count)
+    step = max(1, count // 40)
+    sample_idx = list(range(0, count, step))
+    base = serve.one_at_a_time([reqs[i] for i in sample_idx])
+    mean_s = float(base["service"].mean())
+    mu0 = 1.0 / mean_s
+    by_L = {}
+    for i, s in zip(sample_idx, base["service"]):
+        by_L.setdefault(reqs[i][0], []).append(s)
+    model = np.array([np.mean(by_L[L]) if L in by_L else mean_s for L, _, _ in reqs])
+    log(f"serving: one-at-a-time FP64 {mu0:.2f} solves/s on {len(sample_idx)} sampled requests")
+    out = {"requests": count, "precision": precision, "one_at_a_time_fp64_solves_per_s": round(mu0, 3),
+           "one_at_a_time_sample": len(sample_idx), "rates": []}
+    for mult in rates:
+        lam = mult * mu0
+        arr = poisson_arrivals(count, lam)
+        res = serve.run_stream(reqs, arr, precision)
+        fifo = serve.fifo_latencies(arr, model)
+        dE = float(np.abs(res["energy"][sample_idx] - base["energy"]).max())
+        rec = {"lambda_over_mu0": mult, "lambda": round(lam, 3),
+               "solves_per_s": round(res["solves_per_s"], 3),
+               "speedup_vs_one_at_a_time": round(res["solves_per_s"] / mu0, 2),
+               "p50_s": round(res["p50"], 4), "p99_s": round(res["p99"], 4),
+               "fifo_one_at_a_time_p50_s": round(float(np.percentile(fifo, 50)), 4),
+               "fifo_one_at_a_time_p99_s": round(float(np.percentile(fifo, 99)), 4),
+               "peak_running": res["peak_running"], "all_converged": bool((res["status"] == 2).all()),
+               "max_abs_dE_vs_single_sample": dE}
+        out["rates"].append(rec)
+        log(f"serving lam={mult}x: {res['solves_per_s']:.2f} solves/s "
+            f"({rec['speedup_vs_one_at_a_time']}x), p50 {res['p50']:.3f} s, p99 {res['p99']:.3f} s, "
+            f"peak running {res['peak_running']}")
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -74,6 +158,13 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--calib", action="store_true",
                     help="first run a 1 GiB device copy (byte-counter calibration)")
+    ap.add_argument("--batch", type=int, default=512,
+                    help="instances of the batched-solve leg (0 skips it)")
+    ap.add_argument("--serve", type=int, default=2000,
+                    help="requests of the serving leg (0 skips it)")
+    ap.add_argument("--serve-rates", default="1,5,10",
+                    help="comma-separated arrival rates as multiples of mu0")
+    ap.add_argument("--serve-precision", default="fp64", choices=["fp64", "mixed"])
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the DOCI solve here is a single-GPU "
@@ -167,8 +258,13 @@ def main() -> int:
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(op, rp.info.iterations, args.cpu_threads)
-    print(json.dumps(out), flush=True)
     op.close()
+    if args.batch:
+        out["batched"] = batched_leg(da, args.batch)
+    if args.serve:
+        rates = [float(x) for x in args.serve_rates.split(",")]
+        out["serving"] = serving_leg(da, args.serve, rates, args.serve_precision)
+    print(json.dumps(out), flush=True)
     return 0
 
 

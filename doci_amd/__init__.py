@@ -70,7 +70,7 @@ EXPORTS = [
     "doci_write_fcidump", "doci_gen_integrals", "doci_gen_vector", "doci_read_inputfile",
     "doci_read_fcidump", "doci_free", "doci_dev_copy", "doci_op_set_kernel",
     "doci_op_slab_split", "doci_op_check_violations", "doci_op_tune", "doci_op_time_sigma",
-    "doci_op_star_split", "doci_batch_defaults", "doci_batch_create", "doci_batch_free",
+    "doci_op_star_split", "doci_op_kernel", "doci_batch_defaults", "doci_batch_create", "doci_batch_free",
     "doci_batch_footprint", "doci_batch_add", "doci_batch_step", "doci_batch_run",
     "doci_batch_result_of", "doci_batch_release", "doci_batch_clock", "doci_batch_stats",
 ]
@@ -123,6 +123,7 @@ def lib():
                                      C.POINTER(C.c_uint), C.POINTER(C.c_uint),
                                      C.POINTER(C.c_size_t)]
     l.doci_op_time_sigma.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_double)]
+    l.doci_op_kernel.argtypes = [vp]
     l.doci_batch_defaults.argtypes = [C.POINTER(BatchOpts)]
     l.doci_batch_create.argtypes = [C.POINTER(BatchOpts), C.POINTER(vp)]
     l.doci_batch_free.argtypes = [vp]
@@ -233,7 +234,8 @@ class Operator:
         self.n = int(lib().doci_op_dim(handle))
         self.L = int(lib().doci_op_L(handle))
         self.N = int(lib().doci_op_N(handle))
-        self.kernel = "star"  # the library's default sigma kernel
+        names = {v: k for k, v in self.KERNELS.items()}
+        self.kernel = names[int(lib().doci_op_kernel(handle))]  # default: by size
 
     @classmethod
     def from_fcidump(cls, path: str) -> "Operator":
@@ -278,8 +280,8 @@ class Operator:
     KERNELS = {"star": 0, "lane": 1, "star-checked": 2, "slab": 3, "slab-checked": 4}
 
     def set_kernel(self, name: str) -> None:
-        """Sigma kernel: "star" (default), "slab" (bit-reproducible), "lane", or a
-        "-checked" variant."""
+        """Sigma kernel: "star" (default from 2^20 determinants), "lane"
+        (default below), "slab" (bit-reproducible), or a "-checked" variant."""
         _check(lib().doci_op_set_kernel(self._h, self.KERNELS[name]), "doci_op_set_kernel")
         self.kernel = name
 

@@ -114,6 +114,19 @@ size_t dk_sigma_star_lds(const dk_tables *t, const dk_stars *st, int prec, int p
 int dk_sigma_star(const dk_tables *t, const dk_stars *st, const double *D, double shift,
                   const void *c, void *sigma, void *Z, int prec, uint32_t *bad, hipStream_t s);
 
+/* Operands of a sigma launch read from device memory when the launch runs
+ * (so a recorded launch follows operands chosen on the device). */
+typedef struct {
+    const void *c;
+    void *sigma;
+    double shift;
+    int32_t active;          /* 0: the launch does nothing */
+    int32_t pad;
+} dk_sigma_io;
+/* dk_sigma_star with c, sigma and shift taken from *io. */
+int dk_sigma_star_io(const dk_tables *t, const dk_stars *st, const double *D,
+                     const dk_sigma_io *io, void *Z, int prec, hipStream_t s);
+
 /* ---- batched multi-instance Davidson -----------------------------------
  * Many independent DOCI instances iterate in lock step; every launch covers
  * all of them.  Streaming launches run one workgroup per tile (a fixed-size
@@ -162,12 +175,10 @@ typedef struct {             /* mutable per instance */
     double defl[DK_BKEEP_MAX][DK_BMAX_VEC];
     double A[DK_BMAX_VEC * DK_BMAX_VEC]; /* projected matrix in the current eigenbasis */
     double Y[DK_BMAX_VEC * DK_BMAX_VEC]; /* eigenvectors, column k = Y[j*DK_BMAX_VEC+k] */
-    const void *sig_c;       /* star-kernel sigma of this instance: in / out / shift */
-    void *sig_out;
-    double sig_shift;
+    dk_sigma_io sio;         /* star-kernel sigma operands of this iteration */
 } dk_bstate;
 
-enum { DK_BPH_A = 0, DK_BPH_B = 1, DK_BPH_C = 2, DK_BPH_D = 3, DK_BPH_E = 4 };
+enum { DK_BPH_A = 0, DK_BPH_B = 1, DK_BPH_C = 2, DK_BPH_D = 3, DK_BPH_E = 4, DK_BPH_A0 = 5 };
 enum { DK_BOP_ORTHO = 0, DK_BOP_ORTHO2 = 1, DK_BOP_SCOL = 2, DK_BOP_DEFLATE = 3,
        DK_BOP_RESID = 4 };
 

@@ -27,13 +27,17 @@ typedef struct {
     int want_mixed;
     int prec;               /* running phase: 1 FP32 storage, 0 FP64 */
     int slot;               /* device instance slot while running, else -1 */
+    int fresh;              /* admitted after the status copy in flight was queued */
     unsigned L, N;
     uint32_t n;
     double core;
-    double *one, *two;      /* raw integrals until admission */
+    char *tab;              /* transformed tables (device layout) until admission */
     void *dmain;            /* tables, D and vector slots */
     double *dx;             /* FP64 Ritz vector (kept until release) */
-    size_t bytes;           /* device bytes of dmain */
+    size_t bytes;           /* device bytes held while running (accounted in mem_used) */
+    doci_op *op;            /* large instances: operator with the star-kernel sigma */
+    const dk_stars *st;
+    void *Z;
     dk_binst bi;
     doci_batch_result res;
 } brec;
@@ -72,9 +76,23 @@ struct doci_batch {
     int iterations, graph_launches;
     double *d_red;
     int64_t *d_idx;
+    char *h_stage;          /* pinned staging of instance tables (reused every step) */
+    size_t stage_cap, stage_used;
+    int inflight;           /* an iteration (and its status copy) is queued */
 };
 
 #define ALIGN256(x) (((size_t)(x) + 255) & ~(size_t)255)
+
+/* DOCI_BATCH_TRACE=1 prints admission / retirement / step timings to stderr. */
+static int trace_on(void)
+{
+    static int v = -1;
+    if (v < 0) {
+        const char *e = getenv("DOCI_BATCH_TRACE");
+        v = e && *e && *e != '0';
+    }
+    return v;
+}
 
 void doci_batch_defaults(doci_batch_opts *o)
 {
@@ -104,10 +122,17 @@ static size_t main_bytes(const doci_batch_opts *o, unsigned L, unsigned N, uint3
     return table_bytes(L, N) + slot_stride(n) /* D */ + 2 * P * slot_stride(n);
 }
 
+/* Large instances get the star-kernel sigma (as the single-instance
+ * operator does from this size on); their slot buffer is below (N + 1) n
+ * entries. */
+static int uses_star(uint32_t n) { return n >= DOCI_STAR_MIN_N; }
+
 size_t doci_batch_footprint(const doci_batch_opts *o, unsigned L, unsigned N)
 {
     const uint64_t n = dh_binom(N, L);
-    return main_bytes(o, L, N, (uint32_t)n) + slot_stride((uint32_t)n);
+    size_t b = main_bytes(o, L, N, (uint32_t)n) + slot_stride((uint32_t)n);
+    if (uses_star((uint32_t)n)) b += sizeof(double) * (size_t)(N + 1) * n;
+    return b;
 }
 
 double doci_batch_clock(const doci_batch *b) { return dh_now() - b->t0; }
@@ -151,6 +176,16 @@ int doci_batch_create(const doci_batch_opts *o_in, doci_batch **out)
         return rc;
     }
     for (int i = 0; i < b->cap; ++i) b->slot_rec[i] = -1;
+    {
+        /* keep freed instance memory in the stream-ordered pool instead of
+         * returning it to the driver at every synchronisation */
+        int dev = 0;
+        hipMemPool_t pool;
+        if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+            uint64_t thr = UINT64_MAX;
+            hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
+        }
+    }
     if (!o.mem_budget) {
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) == hipSuccess) b->o.mem_budget = (size_t)(0.9 * (double)fr);
@@ -174,10 +209,16 @@ static void group_free(bgroup *g)
 static void rec_free_main(doci_batch *b, brec *r)
 {
     if (r->dmain) {
-        hipFree(r->dmain);
+        hipFreeAsync(r->dmain, b->stream);
         r->dmain = NULL;
-        b->mem_used -= r->bytes;
     }
+    if (r->op) {
+        hipStreamSynchronize(b->stream);
+        doci_op_free(r->op);
+        r->op = NULL;
+    }
+    b->mem_used -= r->bytes;
+    r->bytes = 0;
 }
 
 void doci_batch_free(doci_batch *b)
@@ -188,8 +229,7 @@ void doci_batch_free(doci_batch *b)
         brec *r = &b->rec[i];
         rec_free_main(b, r);
         if (r->dx) hipFree(r->dx);
-        free(r->one);
-        free(r->two);
+        free(r->tab);
     }
     free(b->rec);
     for (int g = 0; g < 2; ++g) group_free(&b->grp[g]);
@@ -198,6 +238,8 @@ void doci_batch_free(doci_batch *b)
     if (b->h_stat) hipHostFree(b->h_stat);
     if (b->d_red) hipFree(b->d_red);
     if (b->d_idx) hipFree(b->d_idx);
+    if (b->h_stage) hipHostFree(b->h_stage);
+    if (b->stream) hipStreamSynchronize(b->stream);
     if (b->stream) hipStreamDestroy(b->stream);
     free(b->slot_rec);
     free(b);
@@ -222,16 +264,15 @@ int doci_batch_add(doci_batch *b, unsigned L, unsigned N, double core, const dou
     }
     brec *r = &b->rec[b->nrec];
     memset(r, 0, sizeof *r);
-    const size_t l2 = (size_t)L * L;
-    r->one = malloc(sizeof(double) * l2);
-    r->two = malloc(sizeof(double) * l2 * l2);
-    if (!r->one || !r->two) {
-        free(r->one);
-        free(r->two);
-        return doci_fail(DOCI_E_NOMEM, "instance integrals");
+    /* transform now (request intake): only the O(L^2) tables are kept */
+    const size_t l = L, l2 = l * l;
+    r->tab = malloc(table_bytes(L, N));
+    if (!r->tab) return doci_fail(DOCI_E_NOMEM, "instance tables");
+    double *K = (double *)r->tab, *J = K + l2, *X = J + l2, *h = X + l2;
+    if (dh_build_tables(L, N, one, two, K, J, X, h, (uint32_t *)(h + l)) != DOCI_OK) {
+        free(r->tab);
+        return doci_fail(DOCI_E_NOMEM, "instance tables");
     }
-    memcpy(r->one, one, sizeof(double) * l2);
-    memcpy(r->two, two, sizeof(double) * l2 * l2);
     r->L = L;
     r->N = N;
     r->n = (uint32_t)n;
@@ -243,80 +284,117 @@ int doci_batch_add(doci_batch *b, unsigned L, unsigned N, double core, const dou
     return DOCI_OK;
 }
 
-/* Builds the instance's tables and diagonal on the device and starts its
- * first phase in device slot `slot`. */
+/* Room for `bytes` in the pinned staging buffer (growing it after the
+ * stream has drained the copies that read it). */
+static int stage_reserve(doci_batch *b, size_t bytes, char **p)
+{
+    if (b->stage_used + bytes > b->stage_cap) {
+        HIPCHK(hipStreamSynchronize(b->stream));
+        b->stage_used = 0;
+        if (bytes > b->stage_cap) {
+            size_t nc = b->stage_cap ? 2 * b->stage_cap : (1u << 20);
+            while (nc < bytes) nc *= 2;
+            if (b->h_stage) hipHostFree(b->h_stage);
+            b->h_stage = NULL;
+            b->stage_cap = 0;
+            HIPCHK(hipHostMalloc((void **)&b->h_stage, nc, 0));
+            b->stage_cap = nc;
+        }
+    }
+    *p = b->h_stage + b->stage_used;
+    b->stage_used += ALIGN256(bytes);
+    return DOCI_OK;
+}
+
+/* Builds the instance's tables and diagonal on the device (stream-ordered,
+ * no synchronisation) and starts its first phase in device slot `slot`.
+ * Large instances instead get a full operator with the star-kernel sigma. */
 static int admit(doci_batch *b, int ri, int slot)
 {
     brec *r = &b->rec[ri];
     const unsigned L = r->L, N = r->N;
     const uint32_t n = r->n;
     const size_t l = L, l2 = l * l, tb = table_bytes(L, N), vs = slot_stride(n);
-    double *K = malloc(sizeof(double) * l2), *J = malloc(sizeof(double) * l2),
-           *X = malloc(sizeof(double) * l2), *h = malloc(sizeof(double) * l);
-    uint32_t *bt = malloc(sizeof(uint32_t) * (l + 1) * (N + 2));
-    int rc = DOCI_OK;
-    if (!K || !J || !X || !h || !bt) rc = doci_fail(DOCI_E_NOMEM, "instance tables");
-    if (rc == DOCI_OK && dh_build_tables(L, N, r->one, r->two, K, J, X, h, bt) != DOCI_OK)
-        rc = doci_fail(DOCI_E_NOMEM, "instance tables");
-    r->bytes = main_bytes(&b->o, L, N, n);
+    const size_t P = (size_t)(b->o.max_vec + b->o.keep);
+    dk_binst *bi = &r->bi;
+    memset(bi, 0, sizeof *bi);
+    bi->L = L;
+    bi->N = N;
+    bi->n = n;
     hipError_t e = hipSuccess;
-    if (rc == DOCI_OK) {
-        e = hipMalloc(&r->dmain, r->bytes);
-        if (e == hipSuccess) e = hipMalloc((void **)&r->dx, sizeof(double) * (size_t)n);
-        if (e != hipSuccess) {
-            if (r->dmain) hipFree(r->dmain);
-            r->dmain = NULL;
-            rc = doci_fail(DOCI_E_NOMEM, "instance memory: %s", hipGetErrorString(e));
+    int rc = DOCI_OK;
+    const double tt0 = dh_now();
+    double tt1 = tt0, tt2 = tt0;
+    if (uses_star(n)) {
+        const double *K = (const double *)r->tab;
+        rc = dh_op_from_tables(L, N, r->core, K, K + l2, K + 2 * l2, K + 3 * l2, &r->op);
+        size_t zb = 0;
+        tt1 = dh_now();
+        if (rc == DOCI_OK) rc = dh_stars_prepare(r->op, &r->st, &r->Z, &zb);
+        tt2 = dh_now();
+        if (rc != DOCI_OK) {
+            if (r->op) doci_op_free(r->op);
+            r->op = NULL;
+            return rc;
         }
-    }
-    if (rc == DOCI_OK) {
-        b->mem_used += r->bytes;
+        r->bytes = 2 * P * vs + vs /* D */ + zb;
+        e = hipMallocAsync(&r->dmain, 2 * P * vs, b->stream);
+        bi->lane_sigma = 0;
+        bi->binom = r->op->dt.binom;
+        bi->K = r->op->dt.K;
+        bi->D = r->op->d_D;
+        bi->slots = r->dmain;
+    } else {
+        char *h = NULL;
+        rc = stage_reserve(b, tb, &h);
+        if (rc != DOCI_OK) return rc;
+        memcpy(h, r->tab, tb);
+        r->bytes = main_bytes(&b->o, L, N, n);
+        e = hipMallocAsync(&r->dmain, r->bytes, b->stream);
         char *d = r->dmain;
         double *dK = (double *)d, *dJ = dK + l2, *dX = dJ + l2, *dh = dX + l2;
         uint32_t *db = (uint32_t *)(dh + l);
-        e = hipMemcpy(dK, K, sizeof(double) * l2, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(dJ, J, sizeof(double) * l2, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(dX, X, sizeof(double) * l2, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMemcpy(dh, h, sizeof(double) * l, hipMemcpyHostToDevice);
-        if (e == hipSuccess)
-            e = hipMemcpy(db, bt, sizeof(uint32_t) * (l + 1) * (N + 2), hipMemcpyHostToDevice);
-        dk_tables dt = {L, N, n, r->core, db, dK, dJ, dX, dh};
+        if (e == hipSuccess) e = hipMemcpyAsync(d, h, tb, hipMemcpyHostToDevice, b->stream);
+        const dk_tables dt = {L, N, n, r->core, db, dK, dJ, dX, dh};
         double *D = (double *)(d + tb);
         if (e == hipSuccess) e = (hipError_t)dk_diagonal(&dt, D, b->stream);
-        dk_binst *bi = &r->bi;
-        memset(bi, 0, sizeof *bi);
-        bi->L = L;
-        bi->N = N;
-        bi->n = n;
         bi->lane_sigma = 1;
         bi->binom = db;
         bi->K = dK;
         bi->D = D;
         bi->slots = d + tb + vs;
-        bi->slot_stride = vs;
-        bi->x = r->dx;
-        r->prec = r->want_mixed ? 1 : 0;
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(b->d_inst + slot, bi, sizeof *bi, hipMemcpyHostToDevice, b->stream);
-        if (e == hipSuccess)
-            e = (hipError_t)dk_batch_start(bi, b->d_state + slot, 0, r->prec, r->prec,
-                                           r->prec ? b->o.switch_tol : b->o.tol,
-                                           r->prec ? 1e-6 : 1e-12, b->o.max_its, b->o.keep,
-                                           b->o.max_vec, 0, b->d_red, b->d_idx, b->stream);
-        /* the host copy of bi is read by the async copy: finish it here */
-        if (e == hipSuccess) e = hipStreamSynchronize(b->stream);
-        if (e != hipSuccess) rc = doci_fail(DOCI_E_HIP, "instance start: %s", hipGetErrorString(e));
     }
-    free(K);
-    free(J);
-    free(X);
-    free(h);
-    free(bt);
+    if (e == hipSuccess) e = hipMallocAsync((void **)&r->dx, sizeof(double) * (size_t)n, b->stream);
+    if (e != hipSuccess) {
+        if (r->dmain) hipFreeAsync(r->dmain, b->stream);
+        r->dmain = NULL;
+        if (r->op) doci_op_free(r->op);
+        r->op = NULL;
+        return doci_fail(DOCI_E_NOMEM, "instance memory: %s", hipGetErrorString(e));
+    }
+    b->mem_used += r->bytes;
+    bi->slot_stride = vs;
+    bi->x = r->dx;
+    r->prec = r->want_mixed ? 1 : 0;
+    /* the descriptor is staged too: r->bi may move before the copy runs */
+    char *hb = NULL;
+    rc = stage_reserve(b, sizeof *bi, &hb);
     if (rc != DOCI_OK) return rc;
-    free(r->one);
-    free(r->two);
-    r->one = r->two = NULL;
-    memset(&b->h_stat[slot], 0, sizeof(dk_bstat));
+    memcpy(hb, bi, sizeof *bi);
+    e = hipMemcpyAsync(b->d_inst + slot, hb, sizeof *bi, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess)
+        e = (hipError_t)dk_batch_start(bi, b->d_state + slot, 0, r->prec, r->prec,
+                                       r->prec ? b->o.switch_tol : b->o.tol,
+                                       r->prec ? 1e-6 : 1e-12, b->o.max_its, b->o.keep,
+                                       b->o.max_vec, 0, b->d_red, b->d_idx, b->stream);
+    if (e != hipSuccess) return doci_fail(DOCI_E_HIP, "instance start: %s", hipGetErrorString(e));
+    free(r->tab);
+    r->tab = NULL;
+    if (trace_on())
+        fprintf(stderr, "[batch] admit L=%u N=%u n=%u: op %.2f ms, star tables %.2f ms, rest %.2f ms\n",
+                L, N, n, 1e3 * (tt1 - tt0), 1e3 * (tt2 - tt1), 1e3 * (dh_now() - tt2));
+    r->fresh = b->inflight;
+    if (!b->inflight) memset(&b->h_stat[slot], 0, sizeof(dk_bstat));
     r->slot = slot;
     r->state = 1;
     r->res.status = 1;
@@ -340,7 +418,7 @@ static int admit_queued(doci_batch *b)
     for (int ri = b->qhead; ri < b->nrec; ++ri) {
         brec *r = &b->rec[ri];
         if (r->state != 0) continue;
-        const size_t need = main_bytes(&b->o, r->L, r->N, r->n);
+        const size_t need = doci_batch_footprint(&b->o, r->L, r->N);
         if (b->mem_used + need > b->o.mem_budget) break;
         const int slot = free_slot(b);
         if (slot < 0) break;
@@ -413,8 +491,8 @@ static int rebuild(doci_batch *b)
                 g->h_tiles[2 * (t + k) + 1] = k;
             }
             t += r->bi.ntiles;
-            if (r->L > g->maxL) g->maxL = r->L;
-            if (r->N > g->maxN) g->maxN = r->N;
+            if (r->bi.lane_sigma && r->L > g->maxL) g->maxL = r->L;
+            if (r->bi.lane_sigma && r->N > g->maxN) g->maxN = r->N;
             HIPCHK(hipMemcpyAsync(b->d_inst + s, &r->bi, sizeof r->bi, hipMemcpyHostToDevice, b->stream));
         }
         g->ntile = t;
@@ -425,8 +503,8 @@ static int rebuild(doci_batch *b)
                                   hipMemcpyHostToDevice, b->stream));
         }
     }
-    /* host arrays are sources of async copies: complete them */
-    HIPCHK(hipStreamSynchronize(b->stream));
+    /* the host lists are sources of async copies: they are rewritten only by
+     * the next rebuild, which runs after the stream has drained */
     b->dirty = 0;
     return DOCI_OK;
 }
@@ -437,17 +515,25 @@ static int enqueue_iteration(doci_batch *b, int p, int M)
     bgroup *g = &b->grp[p];
     const uint32_t rows = (uint32_t)b->tile_rows;
     hipStream_t s = b->stream;
-    DKCHK(dk_batch_small(DK_BPH_A, g->d_list, g->n, b->d_inst, b->d_state, g->d_part, s));
+    DKCHK(dk_batch_small(DK_BPH_A0, g->d_list, g->n, b->d_inst, b->d_state, g->d_part, s));
     DKCHK(dk_batch_stream(DK_BOP_ORTHO, p, M, b->d_inst, b->d_state, g->d_tiles, g->ntile, rows, g->d_part, s));
     DKCHK(dk_batch_small(DK_BPH_B, g->d_list, g->n, b->d_inst, b->d_state, g->d_part, s));
     DKCHK(dk_batch_stream(DK_BOP_ORTHO2, p, M, b->d_inst, b->d_state, g->d_tiles, g->ntile, rows, g->d_part, s));
     DKCHK(dk_batch_small(DK_BPH_C, g->d_list, g->n, b->d_inst, b->d_state, g->d_part, s));
-    DKCHK(dk_batch_sigma_lane(p, g->maxL, g->maxN, b->d_inst, b->d_state, g->d_tiles, g->ntile, rows, s));
+    if (g->maxL)
+        DKCHK(dk_batch_sigma_lane(p, g->maxL, g->maxN, b->d_inst, b->d_state, g->d_tiles, g->ntile, rows, s));
+    for (int k = 0; k < g->n; ++k) { /* large instances: star kernel, operands read on the device */
+        const brec *r = &b->rec[b->slot_rec[g->h_list[k]]];
+        if (r->op)
+            DKCHK(dk_sigma_star_io(&r->op->dt, r->st, r->op->d_D, &b->d_state[g->h_list[k]].sio,
+                                   r->Z, p, s));
+    }
     DKCHK(dk_batch_stream(DK_BOP_SCOL, p, M, b->d_inst, b->d_state, g->d_tiles, g->ntile, rows, g->d_part, s));
     DKCHK(dk_batch_small(DK_BPH_D, g->d_list, g->n, b->d_inst, b->d_state, g->d_part, s));
     DKCHK(dk_batch_stream(DK_BOP_DEFLATE, p, M, b->d_inst, b->d_state, g->d_tiles, g->ntile, rows, g->d_part, s));
     DKCHK(dk_batch_small(DK_BPH_E, g->d_list, g->n, b->d_inst, b->d_state, g->d_part, s));
     DKCHK(dk_batch_stream(DK_BOP_RESID, p, M, b->d_inst, b->d_state, g->d_tiles, g->ntile, rows, g->d_part, s));
+    DKCHK(dk_batch_small(DK_BPH_A, g->d_list, g->n, b->d_inst, b->d_state, g->d_part, s));
     return DOCI_OK;
 }
 
@@ -498,8 +584,9 @@ static int autotune(doci_batch *b)
             for (int p = 0; p < 2; ++p) {
                 bgroup *g = &b->grp[p];
                 if (!g->n) continue;
-                DKCHK(dk_batch_sigma_lane(p, g->maxL, g->maxN, b->d_inst, b->d_state, g->d_tiles,
-                                          g->ntile, (uint32_t)b->tile_rows, b->stream));
+                if (g->maxL)
+                    DKCHK(dk_batch_sigma_lane(p, g->maxL, g->maxN, b->d_inst, b->d_state, g->d_tiles,
+                                              g->ntile, (uint32_t)b->tile_rows, b->stream));
                 DKCHK(dk_batch_stream(DK_BOP_ORTHO, p, 8, b->d_inst, b->d_state, g->d_tiles,
                                       g->ntile, (uint32_t)b->tile_rows, g->d_part, b->stream));
             }
@@ -534,6 +621,7 @@ static int restart_fp64(doci_batch *b, brec *r, const dk_bstat *st)
 
 static void finish(doci_batch *b, brec *r, const dk_bstat *st)
 {
+    const double t0 = dh_now();
     r->res.status = st->status;
     r->res.iterations = st->its;
     if (r->prec) r->res.iterations_fp32 = st->its;
@@ -545,13 +633,57 @@ static void finish(doci_batch *b, brec *r, const dk_bstat *st)
     r->slot = -1;
     rec_free_main(b, r);
     b->dirty = 1;
+    if (trace_on())
+        fprintf(stderr, "[batch] retire L=%u N=%u: %.2f ms\n", r->L, r->N, 1e3 * (dh_now() - t0));
 }
 
+/* Waits for the queued iteration and acts on the statuses it produced. */
+static int drain(doci_batch *b)
+{
+    if (!b->inflight) return DOCI_OK;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->inflight = 0;
+    b->stage_used = 0; /* every staged copy has completed */
+    for (int s = 0; s < b->cap; ++s) {
+        const int ri = b->slot_rec[s];
+        if (ri < 0) continue;
+        brec *r = &b->rec[ri];
+        if (r->fresh) { /* the copied status predates this instance */
+            r->fresh = 0;
+            memset(&b->h_stat[s], 0, sizeof(dk_bstat));
+            continue;
+        }
+        const dk_bstat *st = &b->h_stat[s];
+        if (st->status == DK_B_ACTIVE) continue;
+        if (r->prec == 1 && st->status == DK_B_CONVERGED) {
+            const int rc = restart_fp64(b, r, st);
+            if (rc != DOCI_OK) return rc;
+        } else {
+            finish(b, r, st);
+        }
+    }
+    return DOCI_OK;
+}
+
+/* Pipelined: first finish the iteration queued by the previous call, then
+ * queue the next iteration of the running instances, then admit queued
+ * instances (their host-side set-up overlaps that iteration; they join the
+ * next one), and return without waiting, so that host work between calls
+ * (request intake) overlaps the GPU too. */
 int doci_batch_step(doci_batch *b, int *running, int *waiting)
 {
-    int rc = admit_queued(b);
+    const double ts0 = dh_now();
+    int rc = drain(b);
+    const double ts1 = dh_now();
     if (rc != DOCI_OK) return rc;
     int nrun = 0;
+    for (int s = 0; s < b->cap; ++s) nrun += b->slot_rec[s] >= 0;
+    const int admit_first = nrun == 0; /* nothing to overlap with */
+    if (admit_first) {
+        rc = admit_queued(b);
+        if (rc != DOCI_OK) return rc;
+    }
+    nrun = 0;
     for (int s = 0; s < b->cap; ++s) nrun += b->slot_rec[s] >= 0;
     if (nrun) {
         if (!b->tuned) {
@@ -576,22 +708,16 @@ int doci_batch_step(doci_batch *b, int *running, int *waiting)
         ++b->iterations;
         HIPCHK(hipMemcpy2DAsync(b->h_stat, sizeof(dk_bstat), b->d_state, sizeof(dk_bstate),
                                 sizeof(dk_bstat), (size_t)b->cap, hipMemcpyDeviceToHost, b->stream));
-        HIPCHK(hipStreamSynchronize(b->stream));
-        for (int s = 0; s < b->cap; ++s) {
-            const int ri = b->slot_rec[s];
-            if (ri < 0) continue;
-            brec *r = &b->rec[ri];
-            const dk_bstat *st = &b->h_stat[s];
-            if (st->status == DK_B_ACTIVE) continue;
-            if (r->prec == 1 && st->status == DK_B_CONVERGED) {
-                rc = restart_fp64(b, r, st);
-                if (rc != DOCI_OK) return rc;
-            } else {
-                finish(b, r, st);
-            }
-        }
-        HIPCHK(hipStreamSynchronize(b->stream));
+        b->inflight = 1;
     }
+    const double ts2 = dh_now();
+    if (!admit_first) {
+        rc = admit_queued(b);
+        if (rc != DOCI_OK) return rc;
+    }
+    if (trace_on())
+        fprintf(stderr, "[batch] step %d: drain %.2f ms, launch %.2f ms, admit %.2f ms\n",
+                b->iterations, 1e3 * (ts1 - ts0), 1e3 * (ts2 - ts1), 1e3 * (dh_now() - ts2));
     int nr = 0, nw = 0;
     for (int i = 0; i < b->nrec; ++i) {
         nr += b->rec[i].state == 1;
@@ -617,6 +743,7 @@ int doci_batch_result_of(doci_batch *b, int id, doci_batch_result *res, double *
 {
     if (id < 0 || id >= b->nrec) return doci_fail(DOCI_E_ARG, "no instance %d", id);
     brec *r = &b->rec[id];
+    if (b->inflight) HIPCHK(hipStreamSynchronize(b->stream)); /* h_stat is being written */
     if (res) {
         *res = r->res;
         if (r->state == 1) {
@@ -638,7 +765,7 @@ int doci_batch_release(doci_batch *b, int id)
     if (id < 0 || id >= b->nrec) return doci_fail(DOCI_E_ARG, "no instance %d", id);
     brec *r = &b->rec[id];
     if (r->state != 2) return doci_fail(DOCI_E_ARG, "instance %d has not finished", id);
-    if (r->dx) hipFree(r->dx);
+    if (r->dx) hipFreeAsync(r->dx, b->stream);
     r->dx = NULL;
     return DOCI_OK;
 }

@@ -9,6 +9,9 @@
 #include "../dk.h"
 
 #define DOCI_OP_MAGIC 0x444f4349414d4431ull /* "DOCIAMD1" */
+/* Below this many determinants the lane kernel is the default sigma (no
+ * decomposition tables to build); from it on, the star kernel. */
+#define DOCI_STAR_MIN_N (1u << 20)
 
 struct doci_op {
     uint64_t magic;
@@ -68,6 +71,9 @@ int dh_is_op(const void *p);
  * raw integrals, transformed as src/doci.c:630-683. */
 int dh_build_tables(unsigned L, unsigned N, const double *one_raw, const double *two_raw,
                     double *K, double *J, double *X, double *h, uint32_t *binom);
+/* Operator from tables already transformed (dh_build_tables). */
+int dh_op_from_tables(unsigned L, unsigned N, double core, const double *K, const double *J,
+                      const double *X, const double *h, doci_op **out);
 
 /* dense.c: small dense eigensolvers */
 void dh_jacobi(int m, double *A, int lda, double *w, double *Z, int ldz);
@@ -115,6 +121,8 @@ void dh_stars_free(doci_op *op);
 void dh_stars_limits(const doci_op *op, unsigned *lds_budget, unsigned *sgmax, unsigned *main_emax);
 int dh_stars_sigma(doci_op *op, const void *in, void *out, int prec, double shift, int checked);
 int64_t dh_stars_violations(doci_op *op);
+/* Star tables and slot buffer of op, built / allocated if missing. */
+int dh_stars_prepare(doci_op *op, const dk_stars **st, void **Z, size_t *zbytes);
 void dh_stars_info(const doci_op *op, unsigned *a, unsigned *b, unsigned *ngroup,
                    unsigned *nsgroup, size_t *zelems);
 

@@ -90,9 +90,8 @@ int dh_build_tables(unsigned L, unsigned N, const double *one_raw, const double 
     return DOCI_OK;
 }
 
-/* Takes ownership of nothing: one/two are transformed in a private copy. */
-int doci_op_from_integrals(unsigned L, unsigned N, double core, const double *one_raw,
-                           const double *two_raw, doci_op **out)
+/* Allocates an operator with empty host tables for (L, N). */
+static int op_alloc(unsigned L, unsigned N, double core, doci_op **out)
 {
     *out = NULL;
     if (L == 0 || L > DK_MAX_L) return doci_fail(DOCI_E_ORBITALS, "Too many orbitals (L=%u).", L);
@@ -118,18 +117,68 @@ int doci_op_from_integrals(unsigned L, unsigned N, double core, const double *on
         doci_op_free(op);
         return doci_fail(DOCI_E_NOMEM, "host tables");
     }
+    *out = op;
+    return DOCI_OK;
+}
+
+static int op_device_setup(doci_op *op);
+
+/* Takes ownership of nothing: one/two are transformed in a private copy. */
+int doci_op_from_integrals(unsigned L, unsigned N, double core, const double *one_raw,
+                           const double *two_raw, doci_op **out)
+{
+    doci_op *op = NULL;
+    int rc = op_alloc(L, N, core, &op);
+    if (rc != DOCI_OK) return rc;
     if (dh_build_tables(L, N, one_raw, two_raw, op->K, op->J, op->X, op->h, op->binom) != DOCI_OK) {
         doci_op_free(op);
         return doci_fail(DOCI_E_NOMEM, "host tables");
     }
+    rc = op_device_setup(op);
+    if (rc != DOCI_OK) {
+        doci_op_free(op);
+        return rc;
+    }
+    *out = op;
+    return DOCI_OK;
+}
 
+int dh_op_from_tables(unsigned L, unsigned N, double core, const double *K, const double *J,
+                      const double *X, const double *h, doci_op **out)
+{
+    doci_op *op = NULL;
+    int rc = op_alloc(L, N, core, &op);
+    if (rc != DOCI_OK) return rc;
+    const size_t l = L, l2 = l * l;
+    const int N2 = (int)N + 2;
+    memcpy(op->K, K, sizeof(double) * l2);
+    memcpy(op->J, J, sizeof(double) * l2);
+    memcpy(op->X, X, sizeof(double) * l2);
+    memcpy(op->h, h, sizeof(double) * l);
+    for (size_t x = 0; x <= l; ++x)
+        for (int j = 0; j < N2; ++j) op->binom[x * N2 + j] = (uint32_t)dh_binom((unsigned)j, (unsigned)x);
+    rc = op_device_setup(op);
+    if (rc != DOCI_OK) {
+        doci_op_free(op);
+        return rc;
+    }
+    *out = op;
+    return DOCI_OK;
+}
+
+/* Uploads the host tables, computes the diagonal. */
+static int op_device_setup(doci_op *op)
+{
+    const unsigned L = op->L, N = op->N;
+    const uint64_t n = (uint64_t)op->n;
+    const double core = op->core;
+    const size_t l = L, l2 = l * l;
+    const int N2 = (int)N + 2;
     /* device side */
     int rc = DOCI_OK;
     int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
-        doci_op_free(op);
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return doci_fail(DOCI_E_HIP, "no HIP device available");
-    }
 #define TRY(expr)                                                                     \
     do {                                                                              \
         hipError_t e_ = (expr);                                                       \
@@ -169,14 +218,10 @@ int doci_op_from_integrals(unsigned L, unsigned N, double core, const double *on
     TRY((hipError_t)dk_diagonal(&op->dt, op->d_D, op->stream));
     TRY(hipStreamSynchronize(op->stream));
 #undef TRY
-    rc = dh_slabs_build(op, 0, 0, 0);
-    if (rc == DOCI_OK) rc = dh_stars_build(op, 0, 0, 0);
-    if (rc != DOCI_OK) goto fail;
-    op->sigma_kind = 0; /* star kernel: the fastest measured at L=34, N=10 */
-    *out = op;
+    /* star / slab tables are built on first use of those kernels */
+    op->sigma_kind = n >= DOCI_STAR_MIN_N ? 0 : 1;
     return DOCI_OK;
 fail:
-    doci_op_free(op);
     return rc;
 }
 
@@ -219,9 +264,19 @@ int doci_op_diagonal(doci_op *op, double *out)
     return DOCI_OK;
 }
 
+/* Builds the tables kernel `kind` needs, once. */
+static int ensure_tables(doci_op *op, int kind)
+{
+    if ((kind == 0 || kind == 2) && !op->stars) return dh_stars_build(op, 0, 0, 0);
+    if ((kind == 3 || kind == 4) && !op->slabs) return dh_slabs_build(op, 0, 0, 0);
+    return DOCI_OK;
+}
+
 int doci_op_sigma_dev(doci_op *op, const void *in, void *out, int prec, double shift)
 {
     if (!in || !out || in == out) return doci_fail(DOCI_E_ARG, "sigma needs distinct buffers");
+    const int rc = ensure_tables(op, op->sigma_kind);
+    if (rc != DOCI_OK) return rc;
     switch (op->sigma_kind) {
     case 0:
     case 2: return dh_stars_sigma(op, in, out, prec, shift, op->sigma_kind == 2);
@@ -236,29 +291,35 @@ int doci_op_sigma_dev(doci_op *op, const void *in, void *out, int prec, double s
 int doci_op_set_kernel(doci_op *op, int kind)
 {
     if (kind < 0 || kind > 4) return doci_fail(DOCI_E_ARG, "kernel kind must be 0..4");
-    if ((kind == 0 || kind == 2) && !op->stars) return doci_fail(DOCI_E_ARG, "no star tables");
-    if (kind >= 3 && !op->slabs) return doci_fail(DOCI_E_ARG, "slab tables unavailable");
+    const int rc = ensure_tables(op, kind);
+    if (rc != DOCI_OK) return rc;
     op->sigma_kind = kind;
     return DOCI_OK;
 }
 
+int doci_op_kernel(const doci_op *op) { return op->sigma_kind; }
+
 int64_t doci_op_check_violations(doci_op *op)
 {
-    const int64_t a = dh_stars_violations(op);
+    const int64_t a = op->stars ? dh_stars_violations(op) : 0;
     if (a < 0) return a;
-    const int64_t b = dh_slabs_violations(op);
+    const int64_t b = op->slabs ? dh_slabs_violations(op) : 0;
     return b < 0 ? b : a + b;
 }
 
-int doci_op_star_split(const doci_op *op, unsigned *a, unsigned *b, unsigned *ngroups,
+int doci_op_star_split(doci_op *op, unsigned *a, unsigned *b, unsigned *ngroups,
                        unsigned *nstargroups, size_t *zslots)
 {
+    const int rc = ensure_tables(op, 0);
+    if (rc != DOCI_OK) return rc;
     dh_stars_info(op, a, b, ngroups, nstargroups, zslots);
-    return op->stars ? DOCI_OK : doci_fail(DOCI_E_ARG, "no star tables");
+    return DOCI_OK;
 }
 
-int doci_op_slab_split(const doci_op *op, unsigned *a, unsigned *b, unsigned *ngroups)
+int doci_op_slab_split(doci_op *op, unsigned *a, unsigned *b, unsigned *ngroups)
 {
+    const int rc = ensure_tables(op, 3);
+    if (rc != DOCI_OK) return rc;
     dh_slabs_info(op, a, b, ngroups);
     return op->slabs ? DOCI_OK : doci_fail(DOCI_E_ARG, "no slab tables");
 }

@@ -219,7 +219,11 @@ int doci_op_tune(doci_op *op, const char *key, int value)
         dh_stars_free(op);
         return dh_stars_build(op, cur[0], cur[1], cur[2]);
     }
-    if (!s) return doci_fail(DOCI_E_ARG, "no slab tables");
+    if (!s) { /* built on first use */
+        const int rc = dh_slabs_build(op, 0, 0, 0);
+        if (rc != DOCI_OK) return rc;
+        s = op->slabs;
+    }
     if (!strcmp(key, "xcd_runs")) {
         s->d.xcd_runs = value != 0;
         return DOCI_OK;

@@ -8,6 +8,7 @@
  * of their A-part, so a slab's Z offset follows from its index in the sector.
  * Main-pass groups pack whole slabs of one sector and are ordered by rank;
  * star groups pack whole stars of one j. */
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -19,14 +20,27 @@
 #define STAR_SGMAX_DEFAULT 64
 #define MAIN_EMAX_DEFAULT 1024  /* raised to the biggest slab */
 
-typedef struct {
+/* Device tables of one shape (L, N, limits).  They do not depend on the
+ * integrals, so every operator of that shape shares them; shapes are built
+ * once per process and kept until it ends. */
+typedef struct star_shape {
+    unsigned L, N;
+    unsigned lim[3]; /* requested LDS budget, stars per group, main emax (0 = default) */
     dk_stars d;
     void *dmem;
-    uint32_t *d_bad;
-    void *dtab;     /* B-mask, hop and insert tables */
-    void *Z;        /* slot buffer, FP64-sized so both precisions fit */
+    void *dtab;      /* B-mask, hop and insert tables */
     size_t zelems;
-    unsigned lim[3]; /* requested LDS budget, stars per group, main emax (0 = default) */
+    struct star_shape *next;
+} star_shape;
+
+static star_shape *g_shapes;
+static pthread_mutex_t g_shapes_lock = PTHREAD_MUTEX_INITIALIZER;
+
+/* Per operator: the shared shape, its own slot buffer and violation counter. */
+typedef struct {
+    const star_shape *sh;
+    void *Z;         /* slot buffer, FP64-sized so both precisions fit */
+    uint32_t *d_bad;
 } dh_stars;
 
 typedef struct {
@@ -58,8 +72,6 @@ void dh_stars_free(doci_op *op)
 {
     dh_stars *s = op->stars;
     if (!s) return;
-    if (s->dmem) hipFree(s->dmem);
-    if (s->dtab) hipFree(s->dtab);
     if (s->d_bad) hipFree(s->d_bad);
     if (s->Z) hipFree(s->Z);
     free(s);
@@ -86,7 +98,7 @@ static uint32_t lex_rank(const int *idx, unsigned m, unsigned b)
 }
 
 /* The B-mask, hop and insert tables of dk_stars, uploaded to s->dtab. */
-static int build_b_tables(dh_stars *s, unsigned b)
+static int build_b_tables(star_shape *s, unsigned b)
 {
     size_t nmask = 0, nhop = 0, nins = 0;
     for (unsigned m = 0; m <= b; ++m) {
@@ -167,7 +179,8 @@ static int build_b_tables(dh_stars *s, unsigned b)
     return DOCI_OK;
 }
 
-int dh_stars_build(doci_op *op, unsigned lds_budget, unsigned sgmax, unsigned main_emax)
+static int build_shape(doci_op *op, unsigned lds_budget, unsigned sgmax, unsigned main_emax,
+                       star_shape **out)
 {
     const size_t STAR_LDS_BUDGET = lds_budget ? lds_budget : STAR_LDS_DEFAULT;
     const unsigned STAR_SGMAX = sgmax ? sgmax : STAR_SGMAX_DEFAULT;
@@ -226,7 +239,7 @@ int dh_stars_build(doci_op *op, unsigned lds_budget, unsigned sgmax, unsigned ma
     sgrp_t *grp = malloc(sizeof(sgrp_t) * (nslab + 1));
     uint64_t *samask = malloc(sizeof(uint64_t) * (nstar + 1));
     uint32_t *sgw = malloc(sizeof(uint32_t) * 2 * (nstar + 1)), *gw = NULL;
-    dh_stars *s = NULL;
+    star_shape *s = NULL;
     int idx[64];
     if (!sstart || !szb || !grp || !samask || !sgw) {
         rc = doci_fail(DOCI_E_NOMEM, "star tables");
@@ -318,8 +331,6 @@ int dh_stars_build(doci_op *op, unsigned lds_budget, unsigned sgmax, unsigned ma
             e = hipMemcpy(d + b_am + b_ss + b_zb + b_g, sgw, b_sg, hipMemcpyHostToDevice);
         if (e == hipSuccess)
             e = hipMemcpy(d + b_am + b_ss + b_zb + b_g + b_sg, zoff, b_zo, hipMemcpyHostToDevice);
-        if (e == hipSuccess) e = hipMalloc((void **)&s->d_bad, sizeof(uint32_t));
-        if (e == hipSuccess) e = hipMemset(s->d_bad, 0, sizeof(uint32_t));
         if (e != hipSuccess) {
             rc = doci_fail(DOCI_E_HIP, "star tables on device: %s", hipGetErrorString(e));
             goto out;
@@ -352,13 +363,14 @@ int dh_stars_build(doci_op *op, unsigned lds_budget, unsigned sgmax, unsigned ma
         rc = doci_fail(DOCI_E_ARG, "star kernels exceed the LDS budget");
         goto out;
     }
-    op->stars = s;
+    s->L = L;
+    s->N = N;
+    *out = s;
     s = NULL;
 out:
     if (s) {
         if (s->dmem) hipFree(s->dmem);
         if (s->dtab) hipFree(s->dtab);
-        if (s->d_bad) hipFree(s->d_bad);
         free(s);
     }
     free(zoff);
@@ -371,19 +383,76 @@ out:
     return rc;
 }
 
+int dh_stars_build(doci_op *op, unsigned lds_budget, unsigned sgmax, unsigned main_emax)
+{
+    const star_shape *sh = NULL;
+    int rc = DOCI_OK;
+    pthread_mutex_lock(&g_shapes_lock);
+    for (const star_shape *c = g_shapes; c; c = c->next)
+        if (c->L == op->L && c->N == op->N && c->lim[0] == lds_budget && c->lim[1] == sgmax &&
+            c->lim[2] == main_emax) {
+            sh = c;
+            break;
+        }
+    if (!sh) {
+        star_shape *ns = NULL;
+        rc = build_shape(op, lds_budget, sgmax, main_emax, &ns);
+        if (rc == DOCI_OK) {
+            ns->next = g_shapes;
+            g_shapes = ns;
+            sh = ns;
+        }
+    }
+    pthread_mutex_unlock(&g_shapes_lock);
+    if (rc != DOCI_OK) return rc;
+    dh_stars *s = calloc(1, sizeof *s);
+    if (!s) return doci_fail(DOCI_E_NOMEM, "star descriptor");
+    s->sh = sh;
+    hipError_t e = hipMalloc((void **)&s->d_bad, sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(s->d_bad, 0, sizeof(uint32_t));
+    if (e != hipSuccess) {
+        if (s->d_bad) hipFree(s->d_bad);
+        free(s);
+        return doci_fail(DOCI_E_HIP, "star counter: %s", hipGetErrorString(e));
+    }
+    op->stars = s;
+    return DOCI_OK;
+}
+
 int dh_stars_sigma(doci_op *op, const void *in, void *out, int prec, double shift, int checked)
 {
     dh_stars *s = op->stars;
     if (!s->Z) { /* the slot buffer is allocated on first use and kept */
-        hipError_t e = hipMalloc(&s->Z, sizeof(double) * (s->zelems ? s->zelems : 1));
+        hipError_t e = hipMalloc(&s->Z, sizeof(double) * (s->sh->zelems ? s->sh->zelems : 1));
         if (e != hipSuccess) {
             s->Z = NULL;
-            return doci_fail(DOCI_E_NOMEM, "sigma slot buffer (%zu entries): %s", s->zelems,
+            return doci_fail(DOCI_E_NOMEM, "sigma slot buffer (%zu entries): %s", s->sh->zelems,
                              hipGetErrorString(e));
         }
     }
-    DKCHK(dk_sigma_star(&op->dt, &s->d, op->d_D, shift, in, out, s->Z, prec,
+    DKCHK(dk_sigma_star(&op->dt, &s->sh->d, op->d_D, shift, in, out, s->Z, prec,
                         checked ? s->d_bad : NULL, op->stream));
+    return DOCI_OK;
+}
+
+int dh_stars_prepare(doci_op *op, const dk_stars **st, void **Z, size_t *zbytes)
+{
+    if (!op->stars) {
+        const int rc = dh_stars_build(op, 0, 0, 0);
+        if (rc != DOCI_OK) return rc;
+    }
+    dh_stars *s = op->stars;
+    if (!s->Z) {
+        hipError_t e = hipMalloc(&s->Z, sizeof(double) * (s->sh->zelems ? s->sh->zelems : 1));
+        if (e != hipSuccess) {
+            s->Z = NULL;
+            return doci_fail(DOCI_E_NOMEM, "sigma slot buffer (%zu entries): %s", s->sh->zelems,
+                             hipGetErrorString(e));
+        }
+    }
+    *st = &s->sh->d;
+    *Z = s->Z;
+    *zbytes = sizeof(double) * (s->sh->zelems ? s->sh->zelems : 1);
     return DOCI_OK;
 }
 
@@ -400,18 +469,18 @@ int64_t dh_stars_violations(doci_op *op)
 void dh_stars_limits(const doci_op *op, unsigned *lds_budget, unsigned *sgmax, unsigned *main_emax)
 {
     const dh_stars *s = op->stars;
-    *lds_budget = s ? s->lim[0] : 0;
-    *sgmax = s ? s->lim[1] : 0;
-    *main_emax = s ? s->lim[2] : 0;
+    *lds_budget = s ? s->sh->lim[0] : 0;
+    *sgmax = s ? s->sh->lim[1] : 0;
+    *main_emax = s ? s->sh->lim[2] : 0;
 }
 
 void dh_stars_info(const doci_op *op, unsigned *a, unsigned *b, unsigned *ngroup,
                    unsigned *nsgroup, size_t *zelems)
 {
     const dh_stars *s = op->stars;
-    *a = s ? s->d.a : 0;
-    *b = s ? s->d.b : 0;
-    *ngroup = s ? s->d.ngroup : 0;
-    *nsgroup = s ? s->d.nsgroup : 0;
-    *zelems = s ? s->zelems : 0;
+    *a = s ? s->sh->d.a : 0;
+    *b = s ? s->sh->d.b : 0;
+    *ngroup = s ? s->sh->d.ngroup : 0;
+    *nsgroup = s ? s->sh->d.nsgroup : 0;
+    *zelems = s ? s->sh->zelems : 0;
 }

@@ -6,8 +6,7 @@
 // instance) that reduce the tile partials in a fixed order and advance that
 // instance's state:
 //
-//   small A   residual norm, loop test `res > tol && its++ < max_its`,
-//             Gram-Schmidt coefficients of t
+//   small A0  (instances just started) loop test with the initial residual
 //   ORTHO     t -= sum g_j V_j  (+ norms and overlaps for the check)
 //   small B   loss of orthogonality -> second pass, or scale of V_m
 //   ORTHO2    second classical Gram-Schmidt pass (flagged instances only)
@@ -21,7 +20,12 @@
 //   small E   basis = the kept Ritz vectors
 //   RESID     x = V y, r = W y - theta x, t = r / |D - E| with the 1e-12
 //             floor (src/davidson.c:214-227), x written in FP64
+//   small A   residual norm, loop test `res > tol && its++ < max_its`,
+//             Gram-Schmidt coefficients of t for the next iteration
 //
+// Every partial sum is written and reduced within one iteration's launch
+// sequence, so the tile layout may change between iterations (instances
+// joining or leaving).
 // Every launch reads each instance's state from device memory, so the same
 // launch sequence serves every iteration and can be captured in a HIP graph.
 // Vectors are stored unnormalised with a scale per basis vector (V_j =
@@ -378,10 +382,14 @@ __global__ __launch_bounds__(kSmall) void k_bsmall(int phase, const uint32_t *__
     const int m = st.st.m;
     const int flags = st.st.flags;
 
-    if (phase == DK_BPH_A) {
-        if (!(flags & DK_BF_FIRST)) reduce_tiles(in, part, m + 2, red);
+    if (phase == DK_BPH_A0 || phase == DK_BPH_A) {
+        // A0 (start of an iteration) handles instances that have just been
+        // started; A (end of an iteration) reduces the residual pass of
+        // everything else, in the same launch sequence that wrote it.
+        const bool first = flags & DK_BF_FIRST;
+        if (first != (phase == DK_BPH_A0)) return;
+        if (!first) reduce_tiles(in, part, m + 2, red);
         if (threadIdx.x == 0) {
-            const bool first = flags & DK_BF_FIRST;
             const double res = first ? st.tol * 10 : sqrt(red[0]); // src/davidson.c:45
             st.st.res = res;
             // src/davidson.c:72: res > tol && its++ < max_its
@@ -397,7 +405,7 @@ __global__ __launch_bounds__(kSmall) void k_bsmall(int phase, const uint32_t *__
             st.st.flags = flags & ~DK_BF_FIRST;
         }
         __syncthreads();
-        if (ctl && !(flags & DK_BF_FIRST))
+        if (ctl && !first)
             for (int j = threadIdx.x; j < m; j += kSmall) st.g[j] = st.s[j] * st.s[j] * red[2 + j];
         return;
     }
@@ -423,7 +431,15 @@ __global__ __launch_bounds__(kSmall) void k_bsmall(int phase, const uint32_t *__
                 st.sin = st.tslot;
                 st.sout = w;
                 st.st.flags = flags & ~DK_BF_ORTHO2;
-                if (w < 0) st.st.status = DK_B_BREAKDOWN;
+                if (w < 0) {
+                    st.st.status = DK_B_BREAKDOWN;
+                } else {
+                    // operands of the star-kernel sigma (instances without lane sigma)
+                    st.sio.c = in.slots + (uint64_t)st.tslot * in.slot_stride;
+                    st.sio.sigma = in.slots + (uint64_t)w * in.slot_stride;
+                    st.sio.shift = st.shift;
+                    st.sio.active = 1;
+                }
             }
         }
         __syncthreads();
@@ -432,6 +448,7 @@ __global__ __launch_bounds__(kSmall) void k_bsmall(int phase, const uint32_t *__
         return;
     }
     if (phase == DK_BPH_D) {
+        if (threadIdx.x == 0) st.sio.active = 0; // this iteration's sigma has run
         reduce_tiles(in, part, m + 1, red);
         const int mn = m + 1;
         // column m of the projected matrix in true (normalised) vectors
@@ -571,6 +588,7 @@ __global__ void k_bstart(dk_bstate *st, const double *__restrict__ D, const int6
     st->shift = shift_min_d ? D[*idx] : 0.0;
     st->theta = 0;
     st->st.energy = st->shift;
+    st->sio.active = 0;
 }
 
 template <typename T, int M>

@@ -126,11 +126,15 @@ __host__ __device__ size_t main_lds(uint32_t L, uint32_t N, const dk_stars &st)
 template <typename T, bool kChecked, int YMAX, int BMAX>
 __global__ __launch_bounds__(kBlock, (YMAX <= 24 ? 3 : 2)) void k_star(dk_tables t, dk_stars st, uint32_t g0,
                                                     const T *__restrict__ c, T *__restrict__ Z,
-                                                    uint32_t *bad)
+                                                    uint32_t *bad, const dk_sigma_io *io)
 {
     extern __shared__ double lds_raw[];
     const uint32_t sg = g0 + blockIdx.x;
     if (sg >= st.nsgroup) return;
+    if (io) {
+        if (!io->active) return;
+        c = (const T *)io->c;
+    }
     StarLds p;
     star_lds<T>(t.L, t.N, st, &p, (char *)lds_raw);
     const int L = (int)t.L, N = (int)t.N, a = (int)st.a, b = (int)st.b;
@@ -270,11 +274,18 @@ __global__ __launch_bounds__(kBlock) void k_star_main(dk_tables t, dk_stars st,
                                                       const double *__restrict__ D, double shift,
                                                       const T *__restrict__ c,
                                                       const T *__restrict__ Z,
-                                                      T *__restrict__ sigma, uint32_t *bad)
+                                                      T *__restrict__ sigma, uint32_t *bad,
+                                                      const dk_sigma_io *io)
 {
     extern __shared__ double lds_raw[];
     const uint32_t g = blockIdx.x;
     if (g >= st.ngroup) return;
+    if (io) {
+        if (!io->active) return;
+        c = (const T *)io->c;
+        sigma = (T *)io->sigma;
+        shift = io->shift;
+    }
     const int L = (int)t.L, N = (int)t.N, a = (int)st.a, b = (int)st.b;
     const uint32_t n = t.n, ztot = st.zoff[a + 1];
     auto ok = [&](uint32_t idx, uint32_t lim) -> bool {
@@ -329,7 +340,7 @@ __global__ __launch_bounds__(kBlock) void k_star_main(dk_tables t, dk_stars st,
 
 template <typename T, bool kChecked>
 int launch(const dk_tables *t, const dk_stars *st, const double *D, double shift, const T *c,
-           T *sigma, T *Z, uint32_t *bad, hipStream_t s)
+           T *sigma, T *Z, uint32_t *bad, const dk_sigma_io *io, hipStream_t s)
 {
     const size_t ls = star_lds<T>(t->L, t->N, *st, nullptr, nullptr);
     if (st->b > 16 || st->a > 64) return (int)hipErrorInvalidValue;
@@ -348,7 +359,7 @@ int launch(const dk_tables *t, const dk_stars *st, const double *D, double shift
 #define DK_STAR_CASE(B)                                                                            \
     case B:                                                                                        \
         hipLaunchKernelGGL((k_star<T, kChecked, 8 * B, 16>), grid, dim3(kBlock), ls, s, *t, *st,  \
-                           g0, c, Z, bad);                                                         \
+                           g0, c, Z, bad, io);                                                     \
         break;
             DK_STAR_CASE(1)
             DK_STAR_CASE(2)
@@ -367,7 +378,7 @@ int launch(const dk_tables *t, const dk_stars *st, const double *D, double shift
     }
     const size_t lm = main_lds<T>(t->L, t->N, *st);
     hipLaunchKernelGGL((k_star_main<T, kChecked>), dim3(st->ngroup), dim3(kBlock), lm, s, *t, *st,
-                       D, shift, c, (const T *)Z, sigma, bad);
+                       D, shift, c, (const T *)Z, sigma, bad, io);
     return (int)hipGetLastError();
 }
 
@@ -388,11 +399,20 @@ extern "C" int dk_sigma_star(const dk_tables *t, const dk_stars *st, const doubl
     if (st->ngroup == 0) return (int)hipErrorInvalidValue;
     if (prec == 0)
         return bad ? launch<double, true>(t, st, D, shift, (const double *)c, (double *)sigma,
-                                          (double *)Z, bad, s)
+                                          (double *)Z, bad, nullptr, s)
                    : launch<double, false>(t, st, D, shift, (const double *)c, (double *)sigma,
-                                           (double *)Z, bad, s);
+                                           (double *)Z, bad, nullptr, s);
     return bad ? launch<float, true>(t, st, D, shift, (const float *)c, (float *)sigma, (float *)Z,
-                                     bad, s)
+                                     bad, nullptr, s)
                : launch<float, false>(t, st, D, shift, (const float *)c, (float *)sigma, (float *)Z,
-                                      bad, s);
+                                      bad, nullptr, s);
+}
+
+extern "C" int dk_sigma_star_io(const dk_tables *t, const dk_stars *st, const double *D,
+                                const dk_sigma_io *io, void *Z, int prec, hipStream_t s)
+{
+    if (st->ngroup == 0 || !io) return (int)hipErrorInvalidValue;
+    if (prec == 0)
+        return launch<double, false>(t, st, D, 0.0, nullptr, nullptr, (double *)Z, nullptr, io, s);
+    return launch<float, false>(t, st, D, 0.0, nullptr, nullptr, (float *)Z, nullptr, io, s);
 }

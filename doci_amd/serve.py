@@ -1,0 +1,109 @@
+"""Serving loop: DOCI solve requests arriving over time, continuously batched
+on one GPU through the batch engine (doci_batch_* in libdoci_amd.so).
+
+Each request carries its integrals in memory (or an FCIDUMP path; both are
+read before the clock starts).  At arrival the loop hands the request to the
+batch, which transforms its integrals (intake); the engine
+starts it at the next iteration boundary once its Davidson subspace fits
+the device-memory budget (FIFO), runs it in lock step with everything else
+that is running, and retires it when it converges.  Time-to-converge of a
+request = its completion time - its arrival time, on the batch's clock.
+
+`one_at_a_time` times the baseline the stream is compared with: the same
+requests solved one after the other as plain FP64 single-instance solves.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import Batch, Operator, gen_integrals, read_fcidump
+
+
+def _integrals(req):
+    if isinstance(req, str):
+        L, Na, Nb, core, one, two = read_fcidump(req)
+        return L, Na, core, one, two
+    L, N, seed = req
+    core, one, two = gen_integrals(L, seed)
+    return L, N, core, one, two
+
+
+def run_stream(requests, arrivals, precision="fp64", **batch_opts) -> dict:
+    """Serve `requests` ((L, N, seed) tuples or FCIDUMP paths) arriving at
+    `arrivals` (s, ascending).  precision: "fp64", "mixed", or a list with
+    one entry per request.  Returns per-request latencies and energies."""
+    R = len(requests)
+    prec = [precision] * R if isinstance(precision, str) else list(precision)
+    # requests carry their integrals: produce them before the clock starts
+    ints = [_integrals(q) for q in requests]
+    b = Batch(**batch_opts)
+    ids = [None] * R
+    t_arr = [0.0] * R
+    nxt = 0
+    t0 = b.clock()
+    peak_running = 0
+    steps = 0
+    while True:
+        now = b.clock() - t0
+        while nxt < R and arrivals[nxt] <= now:
+            L, N, core, one, two = ints[nxt]
+            ids[nxt] = b.add(L, N, core, one, two, prec[nxt])
+            ints[nxt] = None
+            t_arr[nxt] = t0 + arrivals[nxt]
+            nxt += 1
+            now = b.clock() - t0
+        running, waiting = b.step()
+        steps += 1
+        peak_running = max(peak_running, running)
+        if nxt == R and running == 0 and waiting == 0:
+            break
+        if running == 0 and waiting == 0 and nxt < R:
+            # idle: wait for the next arrival
+            delay = arrivals[nxt] - (b.clock() - t0)
+            if delay > 0:
+                time.sleep(min(delay, 0.01))
+    lat, energies, its, status = [], [], [], []
+    t_first, t_last = 1e300, 0.0
+    for r in range(R):
+        res, _ = b.result(ids[r])
+        lat.append(res.t_done - t_arr[r])
+        energies.append(res.energy)
+        its.append(res.iterations)
+        status.append(res.status)
+        t_first = min(t_first, t_arr[r])
+        t_last = max(t_last, res.t_done)
+        b.release(ids[r])
+    st = b.stats()
+    b.close()
+    lat = np.array(lat)
+    return dict(latency=lat, energy=np.array(energies), iterations=np.array(its),
+                status=np.array(status), makespan=t_last - t_first,
+                solves_per_s=R / (t_last - t_first), p50=float(np.percentile(lat, 50)),
+                p99=float(np.percentile(lat, 99)), peak_running=peak_running,
+                steps=steps, tile_rows=st["tile_rows"])
+
+
+def one_at_a_time(requests) -> dict:
+    """Plain FP64 single-instance solves one after the other (operator
+    build from in-memory integrals + Davidson), each timed."""
+    times, energies = [], []
+    for req in requests:
+        L, N, core, one, two = _integrals(req)
+        t = time.perf_counter()
+        with Operator.from_integrals(L, N, core, one, two) as op:
+            r = op.solve("D")
+        times.append(time.perf_counter() - t)
+        energies.append(r.energy)
+    return dict(service=np.array(times), energy=np.array(energies))
+
+
+def fifo_latencies(arrivals, service) -> np.ndarray:
+    """Time-to-converge of a one-at-a-time FIFO queue with these service
+    times: finish_i = max(arrival_i, finish_{i-1}) + service_i."""
+    fin, out = 0.0, []
+    for a, s in zip(arrivals, service):
+        fin = max(a, fin) + s
+        out.append(fin - a)
+    return np.array(out)

@@ -115,19 +115,23 @@ int doci_op_sigma_dev(doci_op *op, const void *in, void *out, int prec, double s
 /* The operator's HIP stream (hipStream_t) and device. */
 void *doci_op_stream(doci_op *op);
 
-/* Sigma kernel of the operator: 0 = two-pass star kernel (the default;
- * it sums a centre's hops with LDS atomics, so results may differ between
- * runs in the last bits), 1 = one lane per determinant, 2 = star kernel with
- * every computed index bounds-checked (violations counted, see
- * doci_op_check_violations, and the access skipped), 3 = one-pass slab
- * kernel (bit-reproducible), 4 = slab kernel bounds-checked. */
+/* Sigma kernel of the operator: 0 = two-pass star kernel (default from
+ * 2^20 determinants on; it sums a centre's hops with LDS atomics, so results
+ * may differ between runs in the last bits), 1 = one lane per determinant
+ * (default below 2^20 determinants), 2 = star kernel with every computed
+ * index bounds-checked (violations counted, see doci_op_check_violations,
+ * and the access skipped), 3 = one-pass slab kernel (bit-reproducible),
+ * 4 = slab kernel bounds-checked.  The star and slab decomposition tables
+ * are built on first use. */
 int doci_op_set_kernel(doci_op *op, int kind);
+/* The operator's current sigma kernel (kinds as above). */
+int doci_op_kernel(const doci_op *op);
 /* The star split: a low / b high orbitals, main-pass and star-pass
  * workgroups, and the number of entries of the slot buffer. */
-int doci_op_star_split(const doci_op *op, unsigned *a, unsigned *b, unsigned *ngroups,
+int doci_op_star_split(doci_op *op, unsigned *a, unsigned *b, unsigned *ngroups,
                        unsigned *nstargroups, size_t *zslots);
 /* The slab kernel's split: a low orbitals, b high orbitals, workgroups. */
-int doci_op_slab_split(const doci_op *op, unsigned *a, unsigned *b, unsigned *ngroups);
+int doci_op_slab_split(doci_op *op, unsigned *a, unsigned *b, unsigned *ngroups);
 /* Index violations counted by kernel kinds 2 and 4 since the operator was built. */
 int64_t doci_op_check_violations(doci_op *op);
 /* Tuning knobs.  Slab kernel: "xcd_runs" (0: groups dispatched in rank

@@ -146,3 +146,45 @@ def test_batch_memory_budget_admission():
         assert peak <= 2
         for i in ids:
             assert b.result(i)[0].status == 2
+
+
+def test_batch_staggered_admission_matches_all_at_once():
+    """Serving pattern: instances join one or two per iteration while others
+    finish (tile layouts change every few iterations); every result equals the
+    all-at-once batch within the solver tolerance."""
+    print("[case] batch staggered admission", flush=True)
+    cases = [(12 + k % 7, 3 + k % 4, 500 + k) for k in range(24)]
+    with da.Batch(tile_rows=512) as b:
+        ids = [b.add_synthetic(*c) for c in cases]
+        b.run()
+        ref = [b.result(i)[0].energy for i in ids]
+    with da.Batch(tile_rows=512) as b:
+        ids = []
+        for k, c in enumerate(cases):
+            ids.append(b.add_synthetic(*c, precision="mixed" if k % 3 == 0 else "fp64"))
+            if k % 2:
+                b.step()
+        b.run()
+        for i, c, e in zip(ids, cases, ref):
+            r = b.result(i)[0]
+            assert r.status == 2 and r.residual <= 1e-8, c
+            assert abs(r.energy - e) <= 1e-9, (c, r.energy - e)
+
+
+def test_batch_large_instances_use_star_kernel():
+    """Instances from 2^20 determinants on take the star-kernel sigma inside
+    the batch (operands read on the device); FP64 and mixed, next to small
+    lane-sigma instances, against independent single solves."""
+    print("[case] batch large instances", flush=True)
+    cases = [(23, 10, 71), (24, 9, 72), (14, 5, 73)]
+    refs = [_single(*c) for c in cases]
+    assert da.binom(10, 23) >= 1 << 20 and da.binom(9, 24) >= 1 << 20
+    for prec in ("fp64", "mixed"):
+        with da.Batch(tile_rows=2048) as b:
+            ids = [b.add_synthetic(*c, precision=prec) for c in cases]
+            b.run()
+            for i, c, ref in zip(ids, cases, refs):
+                r, x = b.result(i, want_x=True)
+                assert r.status == 2 and r.residual <= 1e-8, (prec, c)
+                assert abs(r.energy - ref.energy) <= 1e-9, (prec, c, r.energy - ref.energy)
+                assert abs(abs(x @ ref.x) - 1) <= 1e-8, (prec, c)
