@@ -128,10 +128,12 @@ def serving_leg(da, count: int, rates, precision: str) -> dict:
     log(f"serving: one-at-a-time FP64 {mu0:.2f} solves/s on {len(sample_idx)} sampled requests")
     out = {"requests": count, "precision": precision, "one_at_a_time_fp64_solves_per_s": round(mu0, 3),
            "one_at_a_time_sample": len(sample_idx), "rates": []}
+    ints = [da.gen_integrals(L, s) for L, _, s in reqs]
+    ints = [(L, N) + tuple(g) for (L, N, _), g in zip(reqs, ints)]
     for mult in rates:
         lam = mult * mu0
         arr = poisson_arrivals(count, lam)
-        res = serve.run_stream(reqs, arr, precision)
+        res = serve.run_stream(reqs, arr, precision, integrals=ints)
         fifo = serve.fifo_latencies(arr, model)
         dE = float(np.abs(res["energy"][sample_idx] - base["energy"]).max())
         rec = {"lambda_over_mu0": mult, "lambda": round(lam, 3),

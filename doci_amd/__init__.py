@@ -45,7 +45,8 @@ class SolveInfo(C.Structure):
 class BatchOpts(C.Structure):
     _fields_ = [("tol", C.c_double), ("max_its", C.c_int), ("keep", C.c_int),
                 ("max_vec", C.c_int), ("switch_tol", C.c_double), ("use_graphs", C.c_int),
-                ("tile_rows", C.c_int), ("mem_budget", C.c_size_t), ("max_instances", C.c_int)]
+                ("tile_rows", C.c_int), ("mem_budget", C.c_size_t), ("max_instances", C.c_int),
+                ("keep_vectors", C.c_int), ("mem_reserve", C.c_size_t)]
 
 
 class BatchResult(C.Structure):
@@ -362,13 +363,16 @@ class Batch:
     STATUS = {0: "waiting", 1: "running", 2: "converged", 3: "max_its", 4: "breakdown"}
 
     def __init__(self, tol=1e-8, max_its=200, keep=2, max_vec=30, switch_tol=1e-5,
-                 use_graphs=True, tile_rows=0, mem_budget=0, max_instances=1024):
+                 use_graphs=True, tile_rows=0, mem_budget=0, max_instances=1024,
+                 keep_vectors=True, mem_reserve=0):
         l = lib()
         o = BatchOpts()
         l.doci_batch_defaults(C.byref(o))
         o.tol, o.max_its, o.keep, o.max_vec, o.switch_tol = tol, max_its, keep, max_vec, switch_tol
         o.use_graphs, o.tile_rows, o.mem_budget = int(use_graphs), tile_rows, mem_budget
         o.max_instances = max_instances
+        o.keep_vectors = int(keep_vectors)
+        o.mem_reserve = int(mem_reserve)
         self.opts = o
         h = C.c_void_p()
         _check(l.doci_batch_create(C.byref(o), C.byref(h)), "doci_batch_create")

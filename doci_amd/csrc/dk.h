@@ -129,8 +129,9 @@ int dk_sigma_star_io(const dk_tables *t, const dk_stars *st, const double *D,
 
 /* ---- batched multi-instance Davidson -----------------------------------
  * Many independent DOCI instances iterate in lock step; every launch covers
- * all of them.  Streaming launches run one workgroup per tile (a fixed-size
- * row block of one instance, listed in `tiles`) and write one row of
+ * all of them.  Streaming launches run one workgroup per tile (a row block
+ * of one instance, listed in `tiles`; each instance has its own tile size,
+ * so that large instances use few, long tiles) and write one row of
  * DK_BPV partial sums per tile; "small" launches run one 64-lane workgroup
  * per instance, reduce that instance's tile partials in a fixed order and
  * advance its Davidson state (dk_bstate, device resident).  Each instance's
@@ -143,7 +144,9 @@ int dk_sigma_star_io(const dk_tables *t, const dk_stars *st, const double *D,
 typedef struct {             /* constant per instance */
     uint32_t L, N, n;
     uint32_t tile0, ntiles;  /* its tiles in the tile list */
+    uint32_t tile_rows;      /* rows per tile (ntiles = ceil(n / tile_rows)) */
     uint32_t lane_sigma;     /* 1: sigma by the batched lane kernel */
+    uint32_t pad;
     const uint32_t *binom;   /* (L+1) x (N+2) */
     const double *K;         /* L x L, zero diagonal */
     const double *D;         /* diagonal, n */
@@ -185,15 +188,13 @@ enum { DK_BOP_ORTHO = 0, DK_BOP_ORTHO2 = 1, DK_BOP_SCOL = 2, DK_BOP_DEFLATE = 3,
 /* Streaming step `op` over all tiles; M = largest basis size it may meet
  * (picks the instantiation).  part: ntile x DK_BPV doubles. */
 int dk_batch_stream(int op, int prec, int M, const dk_binst *inst, dk_bstate *state,
-                    const uint32_t *tiles, uint32_t ntile, uint32_t tile_rows, double *part,
-                    hipStream_t s);
+                    const uint32_t *tiles, uint32_t ntile, double *part, hipStream_t s);
 /* Small step `phase` for the ninst instances listed in ilist (device). */
 int dk_batch_small(int phase, const uint32_t *ilist, uint32_t ninst, const dk_binst *inst,
                    dk_bstate *state, const double *part, hipStream_t s);
 /* sigma of every active lane-sigma instance: W[sout] = (H - shift) V[sin]. */
 int dk_batch_sigma_lane(int prec, uint32_t maxL, uint32_t maxN, const dk_binst *inst,
-                        dk_bstate *state, const uint32_t *tiles, uint32_t ntile,
-                        uint32_t tile_rows, hipStream_t s);
+                        dk_bstate *state, const uint32_t *tiles, uint32_t ntile, hipStream_t s);
 /* Admission of instance i (FP64 x, D on device): t = e_argmin(D) (or t = x
  * when from_x), state reset for precision prec; shift = D[argmin] when
  * shift_min_d (else 0).  red: dk_red_scratch_doubles(1) doubles. */

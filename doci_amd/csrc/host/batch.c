@@ -35,9 +35,10 @@ typedef struct {
     void *dmain;            /* tables, D and vector slots */
     double *dx;             /* FP64 Ritz vector (kept until release) */
     size_t bytes;           /* device bytes held while running (accounted in mem_used) */
-    doci_op *op;            /* large instances: operator with the star-kernel sigma */
-    const dk_stars *st;
-    void *Z;
+    size_t xbytes;          /* bytes of dx (accounted until it is freed) */
+    const dk_stars *st;     /* large instances: shared star tables of the shape */
+    void *Z;                /* and their slot buffer (NULL for lane-sigma instances) */
+    size_t zbytes;
     dk_binst bi;
     doci_batch_result res;
 } brec;
@@ -54,6 +55,7 @@ typedef struct {
     hipGraph_t graph;
     hipGraphExec_t gexec;
     int gM;                 /* bucket of the captured graph, 0 = none */
+    int fresh;              /* lists rebuilt since the last iteration */
     uint32_t maxL, maxN;
 } bgroup;
 
@@ -79,6 +81,11 @@ struct doci_batch {
     char *h_stage;          /* pinned staging of instance tables (reused every step) */
     size_t stage_cap, stage_used;
     int inflight;           /* an iteration (and its status copy) is queued */
+    void **chunk;           /* device arena chunks */
+    int nchunk, chunk_cap;
+    struct frange { char *p; size_t bytes; } *frees; /* free ranges of the arena */
+    int nfree, free_cap;
+    size_t arena_bytes;
 };
 
 #define ALIGN256(x) (((size_t)(x) + 255) & ~(size_t)255)
@@ -94,6 +101,8 @@ static int trace_on(void)
     return v;
 }
 
+static int arena_grow(doci_batch *b, size_t bytes);
+
 void doci_batch_defaults(doci_batch_opts *o)
 {
     memset(o, 0, sizeof *o);
@@ -106,6 +115,106 @@ void doci_batch_defaults(doci_batch_opts *o)
     o->tile_rows = 0;
     o->mem_budget = 0;
     o->max_instances = 1024;
+    o->keep_vectors = 1;
+}
+
+/* Device memory arena.  Instance memory is carved from a few large
+ * hipMalloc'd chunks (grown on demand, kept until the batch is freed) by a
+ * first-fit allocator with coalescing, so that admitting and retiring
+ * instances while the GPU runs never calls the driver's allocator (whose
+ * frees synchronise the device).  Ranges are 256-byte aligned. */
+#define ARENA_CHUNK ((size_t)8 << 30)
+
+static int arena_grow(doci_batch *b, size_t bytes)
+{
+    size_t sz = bytes > ARENA_CHUNK ? bytes : ARENA_CHUNK;
+    if (b->arena_bytes + sz > b->o.mem_budget) sz = bytes; /* stay within the budget if possible */
+    if (b->arena_bytes + sz > b->o.mem_budget) return -1;
+    void *p = NULL;
+    if (hipMalloc(&p, sz) != hipSuccess) {
+        (void)hipGetLastError();
+        return -1;
+    }
+    if (b->nchunk == b->chunk_cap || b->nfree == b->free_cap) {
+        const int nc = b->chunk_cap ? 2 * b->chunk_cap : 16, nf = b->free_cap ? 2 * b->free_cap : 256;
+        void **c = realloc(b->chunk, sizeof(void *) * nc);
+        if (c) b->chunk = c, b->chunk_cap = nc;
+        struct frange *f = realloc(b->frees, sizeof *f * nf);
+        if (f) b->frees = f, b->free_cap = nf;
+        if (!c || !f || b->nchunk == b->chunk_cap || b->nfree == b->free_cap) {
+            hipFree(p);
+            return -1;
+        }
+    }
+    b->chunk[b->nchunk] = p;
+    b->frees[b->nfree].p = p;
+    b->frees[b->nfree].bytes = sz;
+    ++b->nfree;
+    ++b->nchunk;
+    b->arena_bytes += sz;
+    return 0;
+}
+
+static void *balloc(doci_batch *b, size_t bytes)
+{
+    bytes = ALIGN256(bytes);
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int i = 0; i < b->nfree; ++i)
+            if (b->frees[i].bytes >= bytes) {
+                char *p = b->frees[i].p;
+                b->frees[i].p = p + bytes;
+                b->frees[i].bytes -= bytes;
+                if (!b->frees[i].bytes) b->frees[i] = b->frees[--b->nfree];
+                return p;
+            }
+        if (pass == 0 && arena_grow(b, bytes) != 0) return NULL;
+    }
+    return NULL;
+}
+
+static void bfree(doci_batch *b, void *vp, size_t bytes)
+{
+    if (!vp) return;
+    char *p = vp;
+    bytes = ALIGN256(bytes);
+    /* merge with adjacent free ranges (chunks are never released while the
+     * batch lives, so a range may also span two neighbouring chunks) */
+    for (int i = 0; i < b->nfree;) {
+        struct frange *f = &b->frees[i];
+        if (f->p + f->bytes == p) {
+            p = f->p;
+            bytes += f->bytes;
+            *f = b->frees[--b->nfree];
+            continue;
+        }
+        if (p + bytes == f->p) {
+            bytes += f->bytes;
+            *f = b->frees[--b->nfree];
+            continue;
+        }
+        ++i;
+    }
+    if (b->nfree == b->free_cap) {
+        const int nf = b->free_cap ? 2 * b->free_cap : 256;
+        struct frange *f = realloc(b->frees, sizeof *f * nf);
+        if (!f) return; /* the range is lost to the arena until the batch is freed */
+        b->frees = f;
+        b->free_cap = nf;
+    }
+    b->frees[b->nfree].p = p;
+    b->frees[b->nfree].bytes = bytes;
+    ++b->nfree;
+}
+
+static void arena_free(doci_batch *b)
+{
+    for (int c = 0; c < b->nchunk; ++c) hipFree(b->chunk[c]);
+    free(b->chunk);
+    free(b->frees);
+    b->chunk = NULL;
+    b->frees = NULL;
+    b->nchunk = b->nfree = b->chunk_cap = b->free_cap = 0;
+    b->arena_bytes = 0;
 }
 
 static size_t table_bytes(unsigned L, unsigned N)
@@ -126,6 +235,16 @@ static size_t main_bytes(const doci_batch_opts *o, unsigned L, unsigned N, uint3
  * operator does from this size on); their slot buffer is below (N + 1) n
  * entries. */
 static int uses_star(uint32_t n) { return n >= DOCI_STAR_MIN_N; }
+
+/* An instance gets at most this many tiles: large instances use long tiles
+ * (few partial sums to reduce, long coalesced streams per workgroup). */
+#define MAX_TILES_PER_INSTANCE 2048u
+
+static uint32_t rows_for(uint32_t n, uint32_t base)
+{
+    const uint32_t cap = (n + MAX_TILES_PER_INSTANCE - 1) / MAX_TILES_PER_INSTANCE;
+    return base > cap ? base : cap;
+}
 
 size_t doci_batch_footprint(const doci_batch_opts *o, unsigned L, unsigned N)
 {
@@ -176,19 +295,19 @@ int doci_batch_create(const doci_batch_opts *o_in, doci_batch **out)
         return rc;
     }
     for (int i = 0; i < b->cap; ++i) b->slot_rec[i] = -1;
-    {
-        /* keep freed instance memory in the stream-ordered pool instead of
-         * returning it to the driver at every synchronisation */
-        int dev = 0;
-        hipMemPool_t pool;
-        if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
-            uint64_t thr = UINT64_MAX;
-            hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
-        }
-    }
+
     if (!o.mem_budget) {
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) == hipSuccess) b->o.mem_budget = (size_t)(0.9 * (double)fr);
+    }
+    if (o.mem_reserve) {
+        /* in 32 GiB chunks (halved when the driver refuses one) */
+        const size_t want = o.mem_reserve < b->o.mem_budget ? o.mem_reserve : b->o.mem_budget;
+        size_t chunk = (size_t)32 << 30;
+        while (b->arena_bytes < want && chunk >= ARENA_CHUNK) {
+            const size_t left = want - b->arena_bytes;
+            if (arena_grow(b, ALIGN256(left < chunk ? left : chunk)) != 0) chunk /= 2;
+        }
     }
     *out = b;
     return DOCI_OK;
@@ -206,17 +325,23 @@ static void group_free(bgroup *g)
     memset(g, 0, sizeof *g);
 }
 
+/* called with the stream drained */
+static void free_x(doci_batch *b, brec *r)
+{
+    bfree(b, r->dx, r->xbytes);
+    r->dx = NULL;
+    b->mem_used -= r->xbytes;
+    r->xbytes = 0;
+}
+
+/* called with the stream drained */
 static void rec_free_main(doci_batch *b, brec *r)
 {
-    if (r->dmain) {
-        hipFreeAsync(r->dmain, b->stream);
-        r->dmain = NULL;
-    }
-    if (r->op) {
-        hipStreamSynchronize(b->stream);
-        doci_op_free(r->op);
-        r->op = NULL;
-    }
+    bfree(b, r->dmain, r->bytes - r->zbytes);
+    r->dmain = NULL;
+    bfree(b, r->Z, r->zbytes);
+    r->Z = NULL;
+    r->zbytes = 0;
     b->mem_used -= r->bytes;
     r->bytes = 0;
 }
@@ -228,10 +353,11 @@ void doci_batch_free(doci_batch *b)
     for (int i = 0; i < b->nrec; ++i) {
         brec *r = &b->rec[i];
         rec_free_main(b, r);
-        if (r->dx) hipFree(r->dx);
+        free_x(b, r);
         free(r->tab);
     }
     free(b->rec);
+    arena_free(b);
     for (int g = 0; g < 2; ++g) group_free(&b->grp[g]);
     if (b->d_inst) hipFree(b->d_inst);
     if (b->d_state) hipFree(b->d_state);
@@ -308,80 +434,81 @@ static int stage_reserve(doci_batch *b, size_t bytes, char **p)
 
 /* Builds the instance's tables and diagonal on the device (stream-ordered,
  * no synchronisation) and starts its first phase in device slot `slot`.
- * Large instances instead get a full operator with the star-kernel sigma. */
+ * Large instances also get the shared star tables of their shape and a
+ * slot buffer, for the star-kernel sigma. */
 static int admit(doci_batch *b, int ri, int slot)
 {
     brec *r = &b->rec[ri];
     const unsigned L = r->L, N = r->N;
     const uint32_t n = r->n;
     const size_t l = L, l2 = l * l, tb = table_bytes(L, N), vs = slot_stride(n);
-    const size_t P = (size_t)(b->o.max_vec + b->o.keep);
+    const double tt0 = dh_now();
     dk_binst *bi = &r->bi;
     memset(bi, 0, sizeof *bi);
     bi->L = L;
     bi->N = N;
     bi->n = n;
-    hipError_t e = hipSuccess;
-    int rc = DOCI_OK;
-    const double tt0 = dh_now();
-    double tt1 = tt0, tt2 = tt0;
+    char *h = NULL;
+    int rc = stage_reserve(b, tb, &h);
+    if (rc != DOCI_OK) return rc;
+    memcpy(h, r->tab, tb);
+    const size_t mb = main_bytes(&b->o, L, N, n);
+    r->dmain = balloc(b, mb);
+    r->dx = balloc(b, sizeof(double) * (size_t)n);
+    if (!r->dmain || !r->dx) {
+        bfree(b, r->dmain, mb);
+        bfree(b, r->dx, sizeof(double) * (size_t)n);
+        r->dmain = NULL;
+        r->dx = NULL;
+        return doci_fail(DOCI_E_NOMEM, "instance memory (L=%u N=%u)", L, N);
+    }
+    char *d = r->dmain;
+    double *dK = (double *)d, *dJ = dK + l2, *dX = dJ + l2, *dh = dX + l2;
+    uint32_t *db = (uint32_t *)(dh + l);
+    const dk_tables dt = {L, N, n, r->core, db, dK, dJ, dX, dh};
+    double *D = (double *)(d + tb);
+    r->zbytes = 0;
+    bi->lane_sigma = 1;
+    const double tt1 = dh_now();
     if (uses_star(n)) {
-        const double *K = (const double *)r->tab;
-        rc = dh_op_from_tables(L, N, r->core, K, K + l2, K + 2 * l2, K + 3 * l2, &r->op);
-        size_t zb = 0;
-        tt1 = dh_now();
-        if (rc == DOCI_OK) rc = dh_stars_prepare(r->op, &r->st, &r->Z, &zb);
-        tt2 = dh_now();
+        size_t zelems = 0;
+        rc = dh_star_shape(&dt, &r->st, &zelems);
+        if (rc == DOCI_OK) {
+            r->zbytes = sizeof(double) * zelems;
+            r->Z = balloc(b, r->zbytes);
+            if (!r->Z) rc = doci_fail(DOCI_E_NOMEM, "slot buffer (L=%u N=%u)", L, N);
+        }
         if (rc != DOCI_OK) {
-            if (r->op) doci_op_free(r->op);
-            r->op = NULL;
+            bfree(b, r->dmain, mb);
+            bfree(b, r->dx, sizeof(double) * (size_t)n);
+            r->dmain = NULL;
+            r->dx = NULL;
+            r->zbytes = 0;
             return rc;
         }
-        r->bytes = 2 * P * vs + vs /* D */ + zb;
-        e = hipMallocAsync(&r->dmain, 2 * P * vs, b->stream);
         bi->lane_sigma = 0;
-        bi->binom = r->op->dt.binom;
-        bi->K = r->op->dt.K;
-        bi->D = r->op->d_D;
-        bi->slots = r->dmain;
-    } else {
-        char *h = NULL;
-        rc = stage_reserve(b, tb, &h);
-        if (rc != DOCI_OK) return rc;
-        memcpy(h, r->tab, tb);
-        r->bytes = main_bytes(&b->o, L, N, n);
-        e = hipMallocAsync(&r->dmain, r->bytes, b->stream);
-        char *d = r->dmain;
-        double *dK = (double *)d, *dJ = dK + l2, *dX = dJ + l2, *dh = dX + l2;
-        uint32_t *db = (uint32_t *)(dh + l);
-        if (e == hipSuccess) e = hipMemcpyAsync(d, h, tb, hipMemcpyHostToDevice, b->stream);
-        const dk_tables dt = {L, N, n, r->core, db, dK, dJ, dX, dh};
-        double *D = (double *)(d + tb);
-        if (e == hipSuccess) e = (hipError_t)dk_diagonal(&dt, D, b->stream);
-        bi->lane_sigma = 1;
-        bi->binom = db;
-        bi->K = dK;
-        bi->D = D;
-        bi->slots = d + tb + vs;
     }
-    if (e == hipSuccess) e = hipMallocAsync((void **)&r->dx, sizeof(double) * (size_t)n, b->stream);
-    if (e != hipSuccess) {
-        if (r->dmain) hipFreeAsync(r->dmain, b->stream);
-        r->dmain = NULL;
-        if (r->op) doci_op_free(r->op);
-        r->op = NULL;
-        return doci_fail(DOCI_E_NOMEM, "instance memory: %s", hipGetErrorString(e));
-    }
-    b->mem_used += r->bytes;
+    const double tt2 = dh_now();
+    r->bytes = mb + r->zbytes;
+    r->xbytes = sizeof(double) * (size_t)n;
+    b->mem_used += r->bytes + r->xbytes;
+    hipError_t e = hipMemcpyAsync(d, h, tb, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) e = (hipError_t)dk_diagonal(&dt, D, b->stream);
+    bi->binom = db;
+    bi->K = dK;
+    bi->D = D;
+    bi->slots = d + tb + vs;
     bi->slot_stride = vs;
     bi->x = r->dx;
     r->prec = r->want_mixed ? 1 : 0;
     /* the descriptor is staged too: r->bi may move before the copy runs */
     char *hb = NULL;
-    rc = stage_reserve(b, sizeof *bi, &hb);
-    if (rc != DOCI_OK) return rc;
-    memcpy(hb, bi, sizeof *bi);
-    e = hipMemcpyAsync(b->d_inst + slot, hb, sizeof *bi, hipMemcpyHostToDevice, b->stream);
+    if (e == hipSuccess) {
+        rc = stage_reserve(b, sizeof *bi, &hb);
+        if (rc != DOCI_OK) return rc;
+        memcpy(hb, bi, sizeof *bi);
+        e = hipMemcpyAsync(b->d_inst + slot, hb, sizeof *bi, hipMemcpyHostToDevice, b->stream);
+    }
     if (e == hipSuccess)
         e = (hipError_t)dk_batch_start(bi, b->d_state + slot, 0, r->prec, r->prec,
                                        r->prec ? b->o.switch_tol : b->o.tol,
@@ -391,7 +518,7 @@ static int admit(doci_batch *b, int ri, int slot)
     free(r->tab);
     r->tab = NULL;
     if (trace_on())
-        fprintf(stderr, "[batch] admit L=%u N=%u n=%u: op %.2f ms, star tables %.2f ms, rest %.2f ms\n",
+        fprintf(stderr, "[batch] admit L=%u N=%u n=%u: memory %.2f ms, star tables %.2f ms, rest %.2f ms\n",
                 L, N, n, 1e3 * (tt1 - tt0), 1e3 * (tt2 - tt1), 1e3 * (dh_now() - tt2));
     r->fresh = b->inflight;
     if (!b->inflight) memset(&b->h_stat[slot], 0, sizeof(dk_bstat));
@@ -414,6 +541,8 @@ static int free_slot(const doci_batch *b)
 /* FIFO admission while the footprint fits the budget. */
 static int admit_queued(doci_batch *b)
 {
+    int running = 0;
+    for (int s = 0; s < b->cap; ++s) running += b->slot_rec[s] >= 0;
     while (b->qhead < b->nrec && b->rec[b->qhead].state != 0) ++b->qhead;
     for (int ri = b->qhead; ri < b->nrec; ++ri) {
         brec *r = &b->rec[ri];
@@ -423,7 +552,9 @@ static int admit_queued(doci_batch *b)
         const int slot = free_slot(b);
         if (slot < 0) break;
         const int rc = admit(b, ri, slot);
+        if (rc == DOCI_E_NOMEM && running > 0) break; /* fragmented: retry when others finish */
         if (rc != DOCI_OK) return rc;
+        ++running;
     }
     return DOCI_OK;
 }
@@ -468,7 +599,8 @@ static int rebuild(doci_batch *b)
             const int ri = b->slot_rec[s];
             if (ri < 0 || b->rec[ri].prec != p) continue;
             ++cnt;
-            ntile += (b->rec[ri].n + rows - 1) / rows;
+            const uint32_t rr = rows_for(b->rec[ri].n, rows);
+            ntile += (b->rec[ri].n + rr - 1) / rr;
         }
         int rc = grow((void **)&g->h_list, (void **)&g->d_list, &g->list_cap, (size_t)cnt + 1,
                       sizeof(uint32_t));
@@ -485,7 +617,8 @@ static int rebuild(doci_batch *b)
             brec *r = &b->rec[ri];
             g->h_list[g->n++] = (uint32_t)s;
             r->bi.tile0 = t;
-            r->bi.ntiles = (r->n + rows - 1) / rows;
+            r->bi.tile_rows = rows_for(r->n, rows);
+            r->bi.ntiles = (r->n + r->bi.tile_rows - 1) / r->bi.tile_rows;
             for (uint32_t k = 0; k < r->bi.ntiles; ++k) {
                 g->h_tiles[2 * (t + k)] = (uint32_t)s;
                 g->h_tiles[2 * (t + k) + 1] = k;
@@ -496,6 +629,7 @@ static int rebuild(doci_batch *b)
             HIPCHK(hipMemcpyAsync(b->d_inst + s, &r->bi, sizeof r->bi, hipMemcpyHostToDevice, b->stream));
         }
         g->ntile = t;
+        g->fresh = 1;
         if (g->n) {
             HIPCHK(hipMemcpyAsync(g->d_list, g->h_list, sizeof(uint32_t) * g->n,
                                   hipMemcpyHostToDevice, b->stream));
@@ -513,26 +647,29 @@ static int rebuild(doci_batch *b)
 static int enqueue_iteration(doci_batch *b, int p, int M)
 {
     bgroup *g = &b->grp[p];
-    const uint32_t rows = (uint32_t)b->tile_rows;
     hipStream_t s = b->stream;
     DKCHK(dk_batch_small(DK_BPH_A0, g->d_list, g->n, b->d_inst, b->d_state, g->d_part, s));
-    DKCHK(dk_batch_stream(DK_BOP_ORTHO, p, M, b->d_inst, b->d_state, g->d_tiles, g->ntile, rows, g->d_part, s));
+    DKCHK(dk_batch_stream(DK_BOP_ORTHO, p, M, b->d_inst, b->d_state, g->d_tiles, g->ntile, g->d_part, s));
     DKCHK(dk_batch_small(DK_BPH_B, g->d_list, g->n, b->d_inst, b->d_state, g->d_part, s));
-    DKCHK(dk_batch_stream(DK_BOP_ORTHO2, p, M, b->d_inst, b->d_state, g->d_tiles, g->ntile, rows, g->d_part, s));
+    DKCHK(dk_batch_stream(DK_BOP_ORTHO2, p, M, b->d_inst, b->d_state, g->d_tiles, g->ntile, g->d_part, s));
     DKCHK(dk_batch_small(DK_BPH_C, g->d_list, g->n, b->d_inst, b->d_state, g->d_part, s));
     if (g->maxL)
-        DKCHK(dk_batch_sigma_lane(p, g->maxL, g->maxN, b->d_inst, b->d_state, g->d_tiles, g->ntile, rows, s));
+        DKCHK(dk_batch_sigma_lane(p, g->maxL, g->maxN, b->d_inst, b->d_state, g->d_tiles, g->ntile, s));
     for (int k = 0; k < g->n; ++k) { /* large instances: star kernel, operands read on the device */
         const brec *r = &b->rec[b->slot_rec[g->h_list[k]]];
-        if (r->op)
-            DKCHK(dk_sigma_star_io(&r->op->dt, r->st, r->op->d_D, &b->d_state[g->h_list[k]].sio,
-                                   r->Z, p, s));
+        if (r->Z) {
+            const size_t l2 = (size_t)r->L * r->L;
+            const double *dK = r->bi.K;
+            const dk_tables dt = {r->L, r->N, r->n, r->core, r->bi.binom, dK, dK + l2, dK + 2 * l2,
+                                  dK + 3 * l2};
+            DKCHK(dk_sigma_star_io(&dt, r->st, r->bi.D, &b->d_state[g->h_list[k]].sio, r->Z, p, s));
+        }
     }
-    DKCHK(dk_batch_stream(DK_BOP_SCOL, p, M, b->d_inst, b->d_state, g->d_tiles, g->ntile, rows, g->d_part, s));
+    DKCHK(dk_batch_stream(DK_BOP_SCOL, p, M, b->d_inst, b->d_state, g->d_tiles, g->ntile, g->d_part, s));
     DKCHK(dk_batch_small(DK_BPH_D, g->d_list, g->n, b->d_inst, b->d_state, g->d_part, s));
-    DKCHK(dk_batch_stream(DK_BOP_DEFLATE, p, M, b->d_inst, b->d_state, g->d_tiles, g->ntile, rows, g->d_part, s));
+    DKCHK(dk_batch_stream(DK_BOP_DEFLATE, p, M, b->d_inst, b->d_state, g->d_tiles, g->ntile, g->d_part, s));
     DKCHK(dk_batch_small(DK_BPH_E, g->d_list, g->n, b->d_inst, b->d_state, g->d_part, s));
-    DKCHK(dk_batch_stream(DK_BOP_RESID, p, M, b->d_inst, b->d_state, g->d_tiles, g->ntile, rows, g->d_part, s));
+    DKCHK(dk_batch_stream(DK_BOP_RESID, p, M, b->d_inst, b->d_state, g->d_tiles, g->ntile, g->d_part, s));
     DKCHK(dk_batch_small(DK_BPH_A, g->d_list, g->n, b->d_inst, b->d_state, g->d_part, s));
     return DOCI_OK;
 }
@@ -540,7 +677,13 @@ static int enqueue_iteration(doci_batch *b, int p, int M)
 static int run_iteration(doci_batch *b, int p, int M)
 {
     bgroup *g = &b->grp[p];
-    if (!b->o.use_graphs) return enqueue_iteration(b, p, M);
+    /* a group that changed since the last iteration runs with direct
+     * launches; it is captured once it stays the same for an iteration, so
+     * that a stream of arrivals does not pay a capture every step */
+    if (!b->o.use_graphs || g->fresh) {
+        g->fresh = 0;
+        return enqueue_iteration(b, p, M);
+    }
     if (!g->gexec || g->gM != M) {
         drop_graph(g);
         HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
@@ -586,9 +729,9 @@ static int autotune(doci_batch *b)
                 if (!g->n) continue;
                 if (g->maxL)
                     DKCHK(dk_batch_sigma_lane(p, g->maxL, g->maxN, b->d_inst, b->d_state, g->d_tiles,
-                                              g->ntile, (uint32_t)b->tile_rows, b->stream));
+                                              g->ntile, b->stream));
                 DKCHK(dk_batch_stream(DK_BOP_ORTHO, p, 8, b->d_inst, b->d_state, g->d_tiles,
-                                      g->ntile, (uint32_t)b->tile_rows, g->d_part, b->stream));
+                                      g->ntile, g->d_part, b->stream));
             }
             HIPCHK(hipEventRecord(e1, b->stream));
             HIPCHK(hipEventSynchronize(e1));
@@ -632,6 +775,7 @@ static void finish(doci_batch *b, brec *r, const dk_bstat *st)
     b->slot_rec[r->slot] = -1;
     r->slot = -1;
     rec_free_main(b, r);
+    if (!b->o.keep_vectors) free_x(b, r);
     b->dirty = 1;
     if (trace_on())
         fprintf(stderr, "[batch] retire L=%u N=%u: %.2f ms\n", r->L, r->N, 1e3 * (dh_now() - t0));
@@ -765,8 +909,8 @@ int doci_batch_release(doci_batch *b, int id)
     if (id < 0 || id >= b->nrec) return doci_fail(DOCI_E_ARG, "no instance %d", id);
     brec *r = &b->rec[id];
     if (r->state != 2) return doci_fail(DOCI_E_ARG, "instance %d has not finished", id);
-    if (r->dx) hipFreeAsync(r->dx, b->stream);
-    r->dx = NULL;
+    if (b->inflight) HIPCHK(hipStreamSynchronize(b->stream));
+    free_x(b, r);
     return DOCI_OK;
 }
 

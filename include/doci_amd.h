@@ -244,6 +244,11 @@ typedef struct {
     int tile_rows;         /* rows per workgroup of the batched launches; 0 = autotune */
     size_t mem_budget;     /* bytes of device memory for running instances (0 = 90% free) */
     int max_instances;     /* capacity (1024) */
+    int keep_vectors;      /* 1: keep a finished instance's eigenvector until
+                              doci_batch_release (counted in the budget); 0: free it
+                              at completion (energies only) (1) */
+    size_t mem_reserve;    /* device bytes taken for the instance arena at creation
+                              (0: grown on demand in 8 GiB chunks) */
 } doci_batch_opts;
 
 typedef struct doci_batch doci_batch;
