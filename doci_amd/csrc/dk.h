@@ -86,6 +86,7 @@ typedef struct {
     uint32_t sgmax;             /* stars per star group, at most */
     uint32_t selems;            /* LDS elements staged per star group, at most */
     uint32_t ymax;              /* largest number of members of a star */
+    uint32_t mfma;              /* 1: star pass on the FP64 matrix cores */
     const uint32_t *slab_start; /* per slab (order: k, then lexicographic alpha) */
     const uint32_t *slab_zbase;
     const uint32_t *group;      /* main groups: first slab, nslab | m << 16 */

@@ -17,6 +17,7 @@
 #define STAR_LDS_DEFAULT (52 * 1024) /* three star-pass workgroups per CU */
 #define MAIN_LDS_MAX (64 * 1024)
 #define STAR_BMAX 16
+#define STAR_MFMA_DEFAULT 0 /* star pass on the matrix cores (tuning key "star_mfma") */
 #define STAR_SGMAX_DEFAULT 64
 #define MAIN_EMAX_DEFAULT 1024  /* raised to the biggest slab */
 
@@ -41,6 +42,7 @@ typedef struct {
     const star_shape *sh;
     void *Z;         /* slot buffer, FP64-sized so both precisions fit */
     uint32_t *d_bad;
+    int mfma;        /* star pass on the matrix cores */
 } dh_stars;
 
 typedef struct {
@@ -345,6 +347,7 @@ static int build_shape(const dk_tables *dt, unsigned lds_budget, unsigned sgmax,
         memcpy(s->d.jgroup, jgroup, sizeof(uint32_t) * (N + 1));
         s->d.selems = selems;
         s->d.ymax = ymax;
+        s->d.mfma = STAR_MFMA_DEFAULT;
         s->d.star_amask = (const uint64_t *)d;
         s->d.slab_start = (const uint32_t *)(d + b_am);
         s->d.slab_zbase = (const uint32_t *)(d + b_am + b_ss);
@@ -428,6 +431,7 @@ int dh_stars_build(doci_op *op, unsigned lds_budget, unsigned sgmax, unsigned ma
     dh_stars *s = calloc(1, sizeof *s);
     if (!s) return doci_fail(DOCI_E_NOMEM, "star descriptor");
     s->sh = sh;
+    s->mfma = (int)sh->d.mfma;
     hipError_t e = hipMalloc((void **)&s->d_bad, sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(s->d_bad, 0, sizeof(uint32_t));
     if (e != hipSuccess) {
@@ -450,8 +454,20 @@ int dh_stars_sigma(doci_op *op, const void *in, void *out, int prec, double shif
                              hipGetErrorString(e));
         }
     }
-    DKCHK(dk_sigma_star(&op->dt, &s->sh->d, op->d_D, shift, in, out, s->Z, prec,
+    dk_stars d = s->sh->d;
+    d.mfma = (uint32_t)s->mfma;
+    DKCHK(dk_sigma_star(&op->dt, &d, op->d_D, shift, in, out, s->Z, prec,
                         checked ? s->d_bad : NULL, op->stream));
+    return DOCI_OK;
+}
+
+int dh_stars_set_mfma(doci_op *op, int on)
+{
+    if (!op->stars) {
+        const int rc = dh_stars_build(op, 0, 0, 0);
+        if (rc != DOCI_OK) return rc;
+    }
+    ((dh_stars *)op->stars)->mfma = on != 0;
     return DOCI_OK;
 }
 

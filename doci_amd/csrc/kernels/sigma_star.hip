@@ -123,28 +123,30 @@ __host__ __device__ size_t main_lds(uint32_t L, uint32_t N, const dk_stars &st)
 // padding, so that they are straight-line code (no per-term branches) whose
 // LDS reads the compiler can batch.  Groups [g0, g0 + gridDim.x) all have
 // a - j <= YMAX.
-template <typename T, bool kChecked, int YMAX, int BMAX>
-__global__ __launch_bounds__(kBlock, (YMAX <= 24 ? 3 : 2)) void k_star(dk_tables t, dk_stars st, uint32_t g0,
-                                                    const T *__restrict__ c, T *__restrict__ Z,
-                                                    uint32_t *bad, const dk_sigma_io *io)
+// What both star-pass kernels know about their star group after set-up.
+struct StarCtx {
+    int L, N, a, b;
+    uint32_t n, ztot;
+    int nst, j, mc, mm, Sm, Sc, ny;
+    bool has_ctr;
+    const uint16_t *ins; // insert table of the members' B-pair count
+};
+
+// Stages K, the binomials, the member tables and the centre slabs of star
+// group sg in LDS (zeroing the centre accumulators); ends with a barrier.
+template <typename T, bool kChecked, typename OK>
+__device__ __forceinline__ StarCtx star_setup(const dk_tables &t, const dk_stars &st, uint32_t sg,
+                                              const T *__restrict__ c, StarLds &p, OK &&ok)
 {
-    extern __shared__ double lds_raw[];
-    const uint32_t sg = g0 + blockIdx.x;
-    if (sg >= st.nsgroup) return;
-    if (io) {
-        if (!io->active) return;
-        c = (const T *)io->c;
-    }
-    StarLds p;
-    star_lds<T>(t.L, t.N, st, &p, (char *)lds_raw);
-    const int L = (int)t.L, N = (int)t.N, a = (int)st.a, b = (int)st.b;
-    const uint32_t n = t.n, ztot = st.zoff[a + 1];
-    auto ok = [&](uint32_t idx, uint32_t lim) -> bool {
-        if (!kChecked) return true;
-        if (idx < lim) return true;
-        atomicAdd(bad, 1u);
-        return false;
-    };
+    StarCtx x;
+    x.L = (int)t.L;
+    x.N = (int)t.N;
+    x.a = (int)st.a;
+    x.b = (int)st.b;
+    x.n = t.n;
+    x.ztot = st.zoff[x.a + 1];
+    const int L = x.L, N = x.N, a = x.a, b = x.b;
+    const uint32_t n = x.n;
     for (int i = threadIdx.x; i < L * L; i += kBlock) p.K[i] = t.K[i];
     for (int i = threadIdx.x; i < (L + 1) * (N + 2); i += kBlock) p.bt[i] = t.binom[i];
     const uint32_t first = st.sgroup[2 * sg], w = st.sgroup[2 * sg + 1];
@@ -156,10 +158,15 @@ __global__ __launch_bounds__(kBlock, (YMAX <= 24 ? 3 : 2)) void k_star(dk_tables
     const int mm = mc - 1;              // B pairs of the members
     const int Sm = (int)C(b, mm), Sc = has_ctr ? (int)C(b, mc) : 0;
     const int ny = a - j;
-    if (kChecked && (ny > YMAX || b > BMAX)) {
-        if (threadIdx.x == 0) atomicAdd(bad, 1u);
-        return;
-    }
+    x.nst = nst;
+    x.j = j;
+    x.mc = mc;
+    x.mm = mm;
+    x.Sm = Sm;
+    x.Sc = Sc;
+    x.ny = ny;
+    x.has_ctr = has_ctr;
+    x.ins = st.ins + st.ins_off[has_ctr ? mm : 0];
     const int km = j + 1;
     const uint32_t zm_sector = st.zoff[km], zc_sector = st.zoff[j];
     const int nsm = nslot_of(km, mm, a), nsc = nslot_of(j, mc, a);
@@ -195,6 +202,52 @@ __global__ __launch_bounds__(kBlock, (YMAX <= 24 ? 3 : 2)) void k_star(dk_tables
             p.acc[e] = 0.0;
         }
     __syncthreads();
+    return x;
+}
+
+// Writes the centre accumulators to their Z slots (after a barrier).
+template <typename T, typename OK>
+__device__ __forceinline__ void star_finish(const StarCtx &x, StarLds &p, T *__restrict__ Z, OK &&ok)
+{
+    if (!x.has_ctr) return;
+    __syncthreads();
+    for (int e = threadIdx.x; e < x.nst * x.Sc; e += kBlock) {
+        const int s = e / x.Sc;
+        const uint32_t zi = p.ctr_zb[s] + (uint32_t)(e - s * x.Sc);
+        if (ok(zi, x.ztot)) Z[zi] = (T)p.acc[e];
+    }
+}
+
+template <typename T, bool kChecked, int YMAX, int BMAX>
+__global__ __launch_bounds__(kBlock, (YMAX <= 24 ? 3 : 2)) void k_star(dk_tables t, dk_stars st, uint32_t g0,
+                                                    const T *__restrict__ c, T *__restrict__ Z,
+                                                    uint32_t *bad, const dk_sigma_io *io)
+{
+    extern __shared__ double lds_raw[];
+    const uint32_t sg = g0 + blockIdx.x;
+    if (sg >= st.nsgroup) return;
+    if (io) {
+        if (!io->active) return;
+        c = (const T *)io->c;
+    }
+    StarLds p;
+    star_lds<T>(t.L, t.N, st, &p, (char *)lds_raw);
+    auto ok = [&](uint32_t idx, uint32_t lim) -> bool {
+        if (!kChecked) return true;
+        if (idx < lim) return true;
+        atomicAdd(bad, 1u);
+        return false;
+    };
+    const StarCtx x = star_setup<T, kChecked>(t, st, sg, c, p, ok);
+    const int L = x.L, a = x.a, b = x.b;
+    const uint32_t n = x.n, ztot = x.ztot;
+    const int nst = x.nst, Sm = x.Sm, Sc = x.Sc, ny = x.ny, mm = x.mm;
+    const bool has_ctr = x.has_ctr;
+    if (kChecked && (ny > YMAX || b > BMAX)) {
+        if (threadIdx.x == 0) atomicAdd(bad, 1u);
+        return;
+    }
+    T *ctrv = (T *)p.ctr;
 
     const double *K = p.K;
     for (int it = threadIdx.x; it < nst * Sm; it += kBlock) {
@@ -259,14 +312,143 @@ __global__ __launch_bounds__(kBlock, (YMAX <= 24 ? 3 : 2)) void k_star(dk_tables
             if (ok(zi, ztot)) Z[zi] = (T)((a0 + a1) + a2);
         }
     }
-    if (has_ctr) {
-        __syncthreads();
-        for (int e = threadIdx.x; e < nst * Sc; e += kBlock) {
-            const int s = e / Sc;
-            const uint32_t zi = p.ctr_zb[s] + (uint32_t)(e - s * Sc);
-            if (ok(zi, ztot)) Z[zi] = (T)p.acc[e];
+    star_finish<T>(x, p, Z, ok);
+}
+
+// Star pass on the matrix cores.  A wave takes one star and 16 member
+// offsets beta at a time and forms, with v_mfma_f64_16x16x4f64 (A 16x4,
+// B 4x16, FP64 accumulate),
+//   x rows:  Z_x[beta] = sum_y K_xy X_y[beta] + sum_t K_xt ctr[lexB(beta+t)]
+//   t rows:  u_t[beta] = sum_y K_ty X_y[beta]  -> centre accumulator at lexB(beta+t)
+// The B operands (member values X_y[beta], centre values at lexB(beta+t))
+// are loaded once per 16 offsets and shared by all row tiles; the A operands
+// (K entries) come from LDS, one read per lane per product.  Lane l holds
+// A[l % 16][l / 16], B[l / 16][l % 16] and D[4 r + l / 16][l % 16] in its
+// r-th accumulator register (layout checked on gfx950).
+// XT: row tiles of 16 members (members <= 16 XT).
+typedef double dk_d4 __attribute__((ext_vector_type(4)));
+
+template <typename T, bool kChecked, int XT>
+__global__ __launch_bounds__(kBlock) void k_star_mfma(dk_tables t, dk_stars st, uint32_t g0,
+                                                      const T *__restrict__ c, T *__restrict__ Z,
+                                                      uint32_t *bad, const dk_sigma_io *io)
+{
+    constexpr int YB = 4 * XT;  // 4-member blocks of the products
+    constexpr int TB = 4;       // 4-orbital blocks of B (b <= 16)
+    constexpr int kWaves = kBlock / 64;
+    extern __shared__ double lds_raw[];
+    const uint32_t sg = g0 + blockIdx.x;
+    if (sg >= st.nsgroup) return;
+    if (io) {
+        if (!io->active) return;
+        c = (const T *)io->c;
+    }
+    StarLds p;
+    star_lds<T>(t.L, t.N, st, &p, (char *)lds_raw);
+    auto ok = [&](uint32_t idx, uint32_t lim) -> bool {
+        if (!kChecked) return true;
+        if (idx < lim) return true;
+        atomicAdd(bad, 1u);
+        return false;
+    };
+    const StarCtx x = star_setup<T, kChecked>(t, st, sg, c, p, ok);
+    const int L = x.L, a = x.a, b = x.b, ny = x.ny, Sm = x.Sm, Sc = x.Sc;
+    if (kChecked && (ny > 16 * XT || b > 4 * TB)) {
+        if (threadIdx.x == 0) atomicAdd(bad, 1u);
+        return;
+    }
+    const T *ctrv = (const T *)p.ctr;
+    const double *K = p.K;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int ny4 = (ny + 3) >> 2, ups = (Sm + 15) >> 4;
+    for (int u = wave; u < x.nst * ups; u += kWaves) {
+        const int s = u / ups;
+        const int beta = (u - s * ups) * 16 + li;
+        const bool bok = beta < Sm;
+        const uint8_t *yc = p.ycol + s * ny;
+        const uint32_t *ms = p.mem_start + s * ny;
+        // B operands: member values and centre values at this lane's (k, beta)
+        double xb[YB], gb[TB];
+#pragma unroll
+        for (int kb = 0; kb < YB; ++kb) {
+            xb[kb] = 0.0;
+            const int y = 4 * kb + lk;
+            if (kb < ny4 && y < ny && bok) {
+                const uint32_t gi = ms[y] + (uint32_t)beta;
+                if (ok(gi, x.n)) xb[kb] = (double)c[gi];
+            }
+        }
+#pragma unroll
+        for (int tb = 0; tb < TB; ++tb) {
+            gb[tb] = 0.0;
+            const int tt = 4 * tb + lk;
+            if (x.has_ctr && tt < b && bok) {
+                const uint32_t off = x.ins[(size_t)tt * Sm + beta];
+                if (off != 0xffffu && ok(off, (uint32_t)Sc)) gb[tb] = (double)ctrv[s * Sc + off];
+            }
+        }
+        // column orbital of each A-operand k index, per member block
+        int ykb[YB];
+#pragma unroll
+        for (int kb = 0; kb < YB; ++kb) {
+            const int y = 4 * kb + lk;
+            ykb[kb] = (kb < ny4 && y < ny) ? yc[y] : -1;
+        }
+        // x rows: members rho + x
+#pragma unroll
+        for (int xt = 0; xt < XT; ++xt) {
+            if (16 * xt >= ny) break; // uniform
+            const int xr = 16 * xt + li;
+            const int xo = xr < ny ? yc[xr] : -1;
+            dk_d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int kb = 0; kb < YB; ++kb) {
+                if (kb >= ny4) break; // uniform
+                const double av = (xo >= 0 && ykb[kb] >= 0) ? K[xo * L + ykb[kb]] : 0.0;
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, xb[kb], acc, 0, 0, 0);
+            }
+            if (x.has_ctr) {
+#pragma unroll
+                for (int tb = 0; tb < TB; ++tb) {
+                    const int tt = 4 * tb + lk;
+                    const double av = (xo >= 0 && tt < b) ? K[xo * L + a + tt] : 0.0;
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, gb[tb], acc, 0, 0, 0);
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int xw = 16 * xt + 4 * r + lk;
+                if (xw < ny && bok) {
+                    const uint32_t zi = p.mem_zb[s * ny + xw] + (uint32_t)beta;
+                    if (ok(zi, x.ztot)) Z[zi] = (T)acc[r];
+                }
+            }
+        }
+        // t rows: the centre's hops from B orbital t into A
+        if (x.has_ctr) {
+            dk_d4 acc = {0.0, 0.0, 0.0, 0.0};
+            const int tr = li;
+#pragma unroll
+            for (int kb = 0; kb < YB; ++kb) {
+                if (kb >= ny4) break; // uniform
+                const double av = (tr < b && ykb[kb] >= 0) ? K[(a + tr) * L + ykb[kb]] : 0.0;
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, xb[kb], acc, 0, 0, 0);
+            }
+            double *accl = p.acc + s * Sc;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int tt = 4 * r + lk;
+                if (tt < b && bok) {
+                    const uint32_t off = x.ins[(size_t)tt * Sm + beta];
+                    if (off != 0xffffu && ok(off, (uint32_t)Sc))
+                        __hip_atomic_fetch_add(&accl[off], acc[r], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            }
         }
     }
+    star_finish<T>(x, p, Z, ok);
 }
 
 template <typename T, bool kChecked>
@@ -344,9 +526,33 @@ int launch(const dk_tables *t, const dk_stars *st, const double *D, double shift
 {
     const size_t ls = star_lds<T>(t->L, t->N, *st, nullptr, nullptr);
     if (st->b > 16 || st->a > 64) return (int)hipErrorInvalidValue;
-    // one launch per run of j sectors sharing a bucket YMAX = 8 * ceil((a - j) / 8)
     const int a = (int)st->a, N = (int)t->N;
     int j = 0;
+    if (st->mfma) {
+        // one launch per run of j sectors sharing the row-tile count ceil((a - j) / 16)
+        while (j < N) {
+            const int xt = (a - j + 15) / 16;
+            int j1 = j + 1;
+            while (j1 < N && (a - j1 + 15) / 16 == xt) ++j1;
+            const uint32_t g0 = st->jgroup[j], g1 = st->jgroup[j1];
+            j = j1;
+            if (g1 <= g0) continue;
+            const dim3 grid(g1 - g0);
+            if (xt <= 1)
+                hipLaunchKernelGGL((k_star_mfma<T, kChecked, 1>), grid, dim3(kBlock), ls, s, *t, *st,
+                                   g0, c, Z, bad, io);
+            else if (xt <= 2)
+                hipLaunchKernelGGL((k_star_mfma<T, kChecked, 2>), grid, dim3(kBlock), ls, s, *t, *st,
+                                   g0, c, Z, bad, io);
+            else
+                hipLaunchKernelGGL((k_star_mfma<T, kChecked, 4>), grid, dim3(kBlock), ls, s, *t, *st,
+                                   g0, c, Z, bad, io);
+            const int e = (int)hipGetLastError();
+            if (e) return e;
+        }
+        j = N; // the vector-FMA path below is skipped
+    }
+    // one launch per run of j sectors sharing a bucket YMAX = 8 * ceil((a - j) / 8)
     while (j < N) {
         const int bucket = (a - j + 7) / 8;
         int j1 = j + 1;

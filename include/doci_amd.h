@@ -137,7 +137,8 @@ int64_t doci_op_check_violations(doci_op *op);
 /* Tuning knobs.  Slab kernel: "xcd_runs" (0: groups dispatched in rank
  * order, 1: contiguous runs per XCD), "emax" (determinants staged per
  * workgroup, rebuilds the slab split), "gmax" (slabs per workgroup).  Star
- * kernel (each rebuilds the star tables; 0 restores the default):
+ * kernel: "star_mfma" (1: star pass on the FP64 matrix cores, 0: vector
+ * FMAs); and (each rebuilds the star tables; 0 restores the default):
  * "star_lds" (LDS bytes of a star-pass workgroup), "star_sgmax" (stars per
  * star-pass workgroup), "star_emax" (determinants staged per main-pass
  * workgroup). */
