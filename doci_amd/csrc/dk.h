@@ -86,7 +86,7 @@ typedef struct {
     uint32_t sgmax;             /* stars per star group, at most */
     uint32_t selems;            /* LDS elements staged per star group, at most */
     uint32_t ymax;              /* largest number of members of a star */
-    uint32_t mfma;              /* 1: star pass on the FP64 matrix cores */
+    uint32_t pass;              /* star pass: 0 column (wave-uniform K), 1 per-star vector, 2 matrix cores */
     const uint32_t *slab_start; /* per slab (order: k, then lexicographic alpha) */
     const uint32_t *slab_zbase;
     const uint32_t *group;      /* main groups: first slab, nslab | m << 16 */
@@ -107,11 +107,15 @@ typedef struct {
     uint32_t bmask_off[17], hop_off[17], ins_off[17];
 } dk_stars;
 
+/* Doubles of packed K behind the slots (0 when the column pass does not apply). */
+size_t dk_star_kp_doubles(uint32_t a, uint32_t b);
 /* LDS bytes of the star pass and of the main pass (prec as for dk_sigma). */
 size_t dk_sigma_star_lds(const dk_tables *t, const dk_stars *st, int prec, int pass);
 /* sigma = (H - shift) c by the star pass then the main pass.  Z: slot buffer
- * of zoff[a+1] elements of the vector precision.  bad: NULL, or a device
- * counter for the bounds-checked instantiation. */
+ * of zoff[a+1] elements of the vector precision, followed (at byte offset
+ * 8 * zoff[a+1]) by dk_star_kp_doubles(a, b) doubles of packed K that the
+ * column star pass writes and reads.  bad: NULL, or a device counter for the
+ * bounds-checked instantiation. */
 int dk_sigma_star(const dk_tables *t, const dk_stars *st, const double *D, double shift,
                   const void *c, void *sigma, void *Z, int prec, uint32_t *bad, hipStream_t s);
 

@@ -121,7 +121,7 @@ void dh_stars_free(doci_op *op);
 void dh_stars_limits(const doci_op *op, unsigned *lds_budget, unsigned *sgmax, unsigned *main_emax);
 int dh_stars_sigma(doci_op *op, const void *in, void *out, int prec, double shift, int checked);
 int64_t dh_stars_violations(doci_op *op);
-int dh_stars_set_mfma(doci_op *op, int on);
+int dh_stars_set_pass(doci_op *op, int pass);
 /* The star tables of shape (dt->L, dt->N) with default limits (shared by
  * every user of that shape, built on first request) and the number of
  * slot-buffer entries a sigma of that shape needs. */

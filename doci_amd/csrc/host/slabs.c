@@ -208,7 +208,7 @@ int dh_slabs_sigma(doci_op *op, const void *in, void *out, int prec, double shif
 int doci_op_tune(doci_op *op, const char *key, int value)
 {
     dh_slabs *s = op->slabs;
-    if (!strcmp(key, "star_mfma")) return dh_stars_set_mfma(op, value);
+    if (!strcmp(key, "star_pass")) return dh_stars_set_pass(op, value);
     if (!strcmp(key, "star_lds") || !strcmp(key, "star_sgmax") || !strcmp(key, "star_emax")) {
         /* rebuild the star tables with one limit changed (0 = default) */
         static const char *keys[3] = {"star_lds", "star_sgmax", "star_emax"};

@@ -17,7 +17,7 @@
 #define STAR_LDS_DEFAULT (52 * 1024) /* three star-pass workgroups per CU */
 #define MAIN_LDS_MAX (64 * 1024)
 #define STAR_BMAX 16
-#define STAR_MFMA_DEFAULT 0 /* star pass on the matrix cores (tuning key "star_mfma") */
+#define STAR_PASS_DEFAULT 0 /* column star pass (tuning key "star_pass") */
 #define STAR_SGMAX_DEFAULT 64
 #define MAIN_EMAX_DEFAULT 1024  /* raised to the biggest slab */
 
@@ -42,7 +42,7 @@ typedef struct {
     const star_shape *sh;
     void *Z;         /* slot buffer, FP64-sized so both precisions fit */
     uint32_t *d_bad;
-    int mfma;        /* star pass on the matrix cores */
+    int pass;        /* star-pass variant (dk_stars.pass) */
 } dh_stars;
 
 typedef struct {
@@ -347,14 +347,14 @@ static int build_shape(const dk_tables *dt, unsigned lds_budget, unsigned sgmax,
         memcpy(s->d.jgroup, jgroup, sizeof(uint32_t) * (N + 1));
         s->d.selems = selems;
         s->d.ymax = ymax;
-        s->d.mfma = STAR_MFMA_DEFAULT;
+        s->d.pass = STAR_PASS_DEFAULT;
         s->d.star_amask = (const uint64_t *)d;
         s->d.slab_start = (const uint32_t *)(d + b_am);
         s->d.slab_zbase = (const uint32_t *)(d + b_am + b_ss);
         s->d.group = (const uint32_t *)(d + b_am + b_ss + b_zb);
         s->d.sgroup = (const uint32_t *)(d + b_am + b_ss + b_zb + b_g);
         s->d.zoff = (const uint32_t *)(d + b_am + b_ss + b_zb + b_g + b_sg);
-        s->zelems = zacc;
+        s->zelems = zacc + dk_star_kp_doubles(a, b); /* slots, then packed K */
         s->lim[0] = lds_budget;
         s->lim[1] = sgmax;
         s->lim[2] = main_emax;
@@ -431,7 +431,7 @@ int dh_stars_build(doci_op *op, unsigned lds_budget, unsigned sgmax, unsigned ma
     dh_stars *s = calloc(1, sizeof *s);
     if (!s) return doci_fail(DOCI_E_NOMEM, "star descriptor");
     s->sh = sh;
-    s->mfma = (int)sh->d.mfma;
+    s->pass = (int)sh->d.pass;
     hipError_t e = hipMalloc((void **)&s->d_bad, sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(s->d_bad, 0, sizeof(uint32_t));
     if (e != hipSuccess) {
@@ -455,19 +455,20 @@ int dh_stars_sigma(doci_op *op, const void *in, void *out, int prec, double shif
         }
     }
     dk_stars d = s->sh->d;
-    d.mfma = (uint32_t)s->mfma;
+    d.pass = (uint32_t)s->pass;
     DKCHK(dk_sigma_star(&op->dt, &d, op->d_D, shift, in, out, s->Z, prec,
                         checked ? s->d_bad : NULL, op->stream));
     return DOCI_OK;
 }
 
-int dh_stars_set_mfma(doci_op *op, int on)
+int dh_stars_set_pass(doci_op *op, int pass)
 {
     if (!op->stars) {
         const int rc = dh_stars_build(op, 0, 0, 0);
         if (rc != DOCI_OK) return rc;
     }
-    ((dh_stars *)op->stars)->mfma = on != 0;
+    if (pass < 0 || pass > 2) return doci_fail(DOCI_E_ARG, "star_pass must be 0, 1 or 2");
+    ((dh_stars *)op->stars)->pass = pass;
     return DOCI_OK;
 }
 

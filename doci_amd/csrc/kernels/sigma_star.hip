@@ -451,6 +451,237 @@ __global__ __launch_bounds__(kBlock) void k_star_mfma(dk_tables t, dk_stars st, 
     star_finish<T>(x, p, Z, ok);
 }
 
+// Column star pass.  With the products written over all of A instead of
+// over each star's members, the K entries no longer depend on the star:
+// for a column (star rho, member offset beta), with X_y = c[rho + y][beta]
+// (zero for y in rho) and C_t = ctr[lexB(beta + t)] (zero for t in beta),
+//   x rows:  Z_x = sum_{y in A} K_xy X_y + sum_{t in B} K_xt C_t   (x not in rho)
+//   t rows:  u_t = sum_{y in A} K_ty X_y  -> centre accumulator at lexB(beta + t)
+// so every lane of a wave runs the same FMA sequence with K as a wave-uniform
+// operand (scalar loads from the packed table KP, built per call by k_kpack),
+// whatever star its column belongs to.  A wave takes 64 consecutive columns
+// of the group; an orbital y that is in rho for every lane's star is skipped
+// by the whole wave (a wave inside one star does (a - j) column passes).
+// AMAX >= a and BMAX >= b pad the row sets; KP is zero in the padding and on
+// the diagonal (x = y is not a hop).
+//   KP[y * (AMAX + BMAX) + r]       r < AMAX: K_{r y};  r >= AMAX: K_{a + r - AMAX, y}
+//   KP[AMAX * (AMAX + BMAX) + q * AMAX + x] = K_{x, a + q}
+__host__ __device__ constexpr int col_amax(int a) { return 8 * ((a + 7) / 8); }
+__host__ __device__ constexpr int col_bmax(int b) { return b <= 12 ? 12 : 16; }
+__host__ __device__ constexpr size_t col_kp_doubles(int am, int bm) { return (size_t)am * (am + bm) + (size_t)bm * am; }
+constexpr int kColAMax = 32; // larger A sets use the per-star kernel
+
+template <int AMAX, int BMAX>
+__global__ __launch_bounds__(kBlock) void k_kpack(dk_tables t, int a, int b, double *__restrict__ KP)
+{
+    constexpr int R = AMAX + BMAX;
+    const int L = (int)t.L;
+    for (int i = threadIdx.x; i < AMAX * R; i += kBlock) {
+        const int y = i / R, r = i - y * R;
+        double v = 0.0;
+        if (y < a) {
+            if (r < AMAX) {
+                if (r < a && r != y) v = t.K[r * L + y];
+            } else if (r - AMAX < b) {
+                v = t.K[(a + r - AMAX) * L + y];
+            }
+        }
+        KP[i] = v;
+    }
+    for (int i = threadIdx.x; i < BMAX * AMAX; i += kBlock) {
+        const int q = i / AMAX, xo = i - q * AMAX;
+        KP[AMAX * R + i] = (q < b && xo < a) ? t.K[xo * L + a + q] : 0.0;
+    }
+}
+
+// An integer laundered through an empty asm tied to a loop-varying value:
+// loads at offsets derived from it cannot be hoisted out of the loop.  The
+// asm has no side effects and touches no memory, so such loads of read-only
+// kernel arguments stay scalar.
+__device__ __forceinline__ int opaque(int v, int tie)
+{
+    v = __builtin_amdgcn_readfirstlane(v);
+    asm("" : "+s"(v) : "s"(__builtin_amdgcn_readfirstlane(tie)));
+    return v;
+}
+
+// out[r] += k[r] * v for r < NR, k in global memory read through scalar
+// registers 8 entries at a time: kbuf holds the first 8 entries of k on
+// entry, and on exit the first 8 of kn (the next call's k), which are loaded
+// while the last chunk's FMAs run.  The scheduling barriers keep one chunk
+// in flight, so at most 16 scalar registers hold K.
+template <int NR, int NOUT>
+__device__ __forceinline__ void col_fma(const double *k, const double *kn, double v, double (&out)[NOUT],
+                                        double (&kbuf)[8])
+{
+    constexpr int NCH = (NR + 7) / 8;
+#pragma unroll
+    for (int ch = 0; ch < NCH; ++ch) {
+        double knx[8];
+        const bool last = ch + 1 == NCH;
+        const double *src = last ? kn : k + 8 * (ch + 1);
+        const int cnt = last ? 8 : (NR - 8 * (ch + 1) < 8 ? NR - 8 * (ch + 1) : 8);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) knx[i] = i < cnt ? src[i] : 0.0;
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+            if (8 * ch + i < NR) out[8 * ch + i] = __builtin_fma(kbuf[i], v, out[8 * ch + i]);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < 8; ++i) kbuf[i] = knx[i];
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ T load_global(const T *p)
+{
+    return *(const __attribute__((address_space(1))) T *)p;
+}
+
+template <typename T, bool kChecked, int AMAX, int BMAX>
+__global__ __launch_bounds__(kBlock, 3) void k_star_col(dk_tables t, dk_stars st, const T *__restrict__ c,
+                                                        T *__restrict__ Z, uint32_t *bad,
+                                                        const dk_sigma_io *io,
+                                                        const double *__restrict__ KP)
+{
+    constexpr int R = AMAX + BMAX;
+    extern __shared__ double lds_raw[];
+    const uint32_t sg = blockIdx.x;
+    if (sg >= st.nsgroup) return;
+    if (io) {
+        if (!io->active) return;
+        c = (const T *)io->c;
+    }
+    StarLds p;
+    star_lds<T>(t.L, t.N, st, &p, (char *)lds_raw);
+    auto ok = [&](uint32_t idx, uint32_t lim) -> bool {
+        if (!kChecked) return true;
+        if (idx < lim) return true;
+        atomicAdd(bad, 1u);
+        return false;
+    };
+    const StarCtx x = star_setup<T, kChecked>(t, st, sg, c, p, ok);
+    // star-group constants (wave-uniform; some were read from LDS)
+    const int a = (int)st.a, b = (int)st.b;
+    const int ny = __builtin_amdgcn_readfirstlane(x.ny);
+    const int Sm = __builtin_amdgcn_readfirstlane(x.Sm);
+    const int Sc = __builtin_amdgcn_readfirstlane(x.Sc);
+    const int nst = __builtin_amdgcn_readfirstlane(x.nst);
+    const bool has_ctr = __builtin_amdgcn_readfirstlane((int)x.has_ctr) != 0;
+    if (a > AMAX || b > BMAX) { // uniform: the launcher picks AMAX >= a, BMAX >= b
+        if (kChecked && threadIdx.x == 0) atomicAdd(bad, 1u);
+        return;
+    }
+    const uint32_t first = st.sgroup[2 * sg];
+    const int ncol = nst * Sm;
+    const uint64_t amask = a < 64 ? ((1ull << a) - 1) : ~0ull;
+    const int lane = (int)(threadIdx.x & 63);
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    for (int c0 = wave * 64; c0 < ncol; c0 += kBlock) {
+        // K addresses recomputed per column block (scalar adds) instead of
+        // hoisted out of the loop into spilled registers
+        const double *kp = KP + opaque(0, c0);
+        const int col = c0 + lane;
+        const bool live = col < ncol;
+        const int s = live ? col / Sm : 0;
+        const int beta = live ? col - s * Sm : 0;
+        const uint64_t rho = live ? st.star_amask[first + s] : amask;
+        // orbitals in rho for every lane's star: skipped by the whole wave
+        uint32_t clo = (uint32_t)rho, chi = (uint32_t)(rho >> 32);
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            clo &= (uint32_t)__shfl_xor((int)clo, o);
+            chi &= (uint32_t)__shfl_xor((int)chi, o);
+        }
+        const uint64_t common = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)chi) << 32) |
+                                (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)clo);
+        const uint32_t *ms = p.mem_start + s * ny;
+        const uint32_t *mz = p.mem_zb + s * ny;
+        // member values X_y, all loads in flight at once, coalesced across the
+        // lanes of one star.  Lanes without member y (y in rho, y >= a, or no
+        // column) read a zero instead: KP[0] is a zero diagonal entry, which
+        // reads as 0 in either precision.
+        const T *zero = (const T *)kp;
+        const uint64_t rhox = live ? (rho | ~amask) : ~0ull;
+        T v[AMAX];
+#pragma unroll
+        for (int y = 0; y < AMAX; ++y) {
+            const bool mem = !((rhox >> y) & 1);
+            const int yi = mem ? y - __popcll(rho & ((1ull << y) - 1)) : 0;
+            const uint32_t gi = ms[yi] + (uint32_t)beta;
+            v[y] = load_global(mem && ok(gi, x.n) ? c + gi : zero);
+        }
+        // out[r]: row r < AMAX is member row x = r, row AMAX + q the centre row
+        // of B orbital a + q
+        double out[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) out[r] = 0.0;
+        const double *kctr = kp + AMAX * R; // centre columns
+        const uint64_t ym = ~common & amask; // the wave's orbitals y
+        double kbuf[8];
+        {
+            const double *k0 = ym ? kp + (__ffsll((unsigned long long)ym) - 1) * R : kctr;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) kbuf[i] = k0[i];
+        }
+#pragma unroll
+        for (int y = 0; y < AMAX; ++y) {
+            if ((ym >> y) & 1) {
+                const uint64_t rest = ym & ~((2ull << y) - 1);
+                const double *kn = rest ? kp + (__ffsll((unsigned long long)rest) - 1) * R : kctr;
+                col_fma<R>(kp + y * R, kn, (double)v[y], out, kbuf);
+            }
+        }
+        // hops A -> B through the centre values, and the centre's hops B -> A:
+        // the insert offsets lexB(beta + t) and centre values first (all loads
+        // in flight together), the accumulator updates, then the products
+        if (has_ctr) {
+            // per-block copies of b and Sm: comparisons and offsets derived
+            // from them are recomputed here instead of held across the loop
+            const int bq = opaque(b, c0), Smq = opaque(Sm, c0);
+            const T *ctrv = (const T *)p.ctr + s * Sc;
+            double *cacc = p.acc + s * Sc;
+            uint32_t offs[BMAX];
+#pragma unroll
+            for (int q = 0; q < BMAX; ++q) {
+                const uint16_t *row = x.ins + (size_t)(q * Smq); // uniform row base
+                offs[q] = q < bq ? (uint32_t)row[(uint32_t)beta] : 0xffffu;
+            }
+            uint32_t on = 0;
+            double cv[BMAX];
+#pragma unroll
+            for (int q = 0; q < BMAX; ++q) {
+                const bool o = live && offs[q] != 0xffffu && ok(offs[q], (uint32_t)Sc);
+                on |= (uint32_t)o << q;
+                const double val = (double)ctrv[o ? offs[q] : 0];
+                cv[q] = o ? val : 0.0;
+            }
+#pragma unroll
+            for (int q = 0; q < BMAX; ++q)
+                if ((on >> q) & 1)
+                    __hip_atomic_fetch_add(&cacc[offs[q]], out[AMAX + q], __ATOMIC_RELAXED,
+                                           __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+            for (int q = 0; q < BMAX; ++q)
+                if (q < bq)
+                    col_fma<AMAX>(kctr + q * AMAX, q + 1 < bq ? kctr + (q + 1) * AMAX : kp, cv[q], out, kbuf);
+        }
+        // member slots
+#pragma unroll
+        for (int xo = 0; xo < AMAX; ++xo) {
+            if (xo < a && !((common >> xo) & 1)) {
+                if (live && !((rho >> xo) & 1)) {
+                    const int xi = xo - __popcll(rho & ((1ull << xo) - 1));
+                    const uint32_t zi = mz[xi] + (uint32_t)beta;
+                    if (ok(zi, x.ztot)) Z[zi] = (T)out[xo];
+                }
+            }
+        }
+    }
+    star_finish<T>(x, p, Z, ok);
+}
+
 template <typename T, bool kChecked>
 __global__ __launch_bounds__(kBlock) void k_star_main(dk_tables t, dk_stars st,
                                                       const double *__restrict__ D, double shift,
@@ -526,9 +757,33 @@ int launch(const dk_tables *t, const dk_stars *st, const double *D, double shift
 {
     const size_t ls = star_lds<T>(t->L, t->N, *st, nullptr, nullptr);
     if (st->b > 16 || st->a > 64) return (int)hipErrorInvalidValue;
-    const int a = (int)st->a, N = (int)t->N;
+    const int a = (int)st->a, b = (int)st->b, N = (int)t->N;
     int j = 0;
-    if (st->mfma) {
+    if (st->pass == 0 && a >= 1 && a <= kColAMax) {
+        // one launch for every star group: K packed behind the slots, then the column pass
+        double *kp = (double *)((char *)Z + sizeof(double) * (size_t)st->zoff[a + 1]);
+        const int am = col_amax(a), bm = col_bmax(b);
+        const dim3 grid(st->nsgroup);
+#define DK_COL_CASE(AM, BM)                                                                        \
+    if (am == AM && bm == BM) {                                                                    \
+        hipLaunchKernelGGL((k_kpack<AM, BM>), dim3(1), dim3(kBlock), 0, s, *t, a, b, kp);          \
+        hipLaunchKernelGGL((k_star_col<T, kChecked, AM, BM>), grid, dim3(kBlock), ls, s, *t, *st, \
+                           c, Z, bad, io, (const double *)kp);                                     \
+    }
+        DK_COL_CASE(8, 12)
+        DK_COL_CASE(8, 16)
+        DK_COL_CASE(16, 12)
+        DK_COL_CASE(16, 16)
+        DK_COL_CASE(24, 12)
+        DK_COL_CASE(24, 16)
+        DK_COL_CASE(32, 12)
+        DK_COL_CASE(32, 16)
+#undef DK_COL_CASE
+        const int e = (int)hipGetLastError();
+        if (e) return e;
+        j = N; // the per-star paths below are skipped
+    }
+    if (st->pass == 2) {
         // one launch per run of j sectors sharing the row-tile count ceil((a - j) / 16)
         while (j < N) {
             const int xt = (a - j + 15) / 16;
@@ -589,6 +844,12 @@ int launch(const dk_tables *t, const dk_stars *st, const double *D, double shift
 }
 
 } // namespace
+
+extern "C" size_t dk_star_kp_doubles(uint32_t a, uint32_t b)
+{
+    if (a < 1 || a > (uint32_t)kColAMax) return 0;
+    return col_kp_doubles(col_amax((int)a), col_bmax((int)b));
+}
 
 extern "C" size_t dk_sigma_star_lds(const dk_tables *t, const dk_stars *st, int prec, int pass)
 {

@@ -127,7 +127,8 @@ int doci_op_set_kernel(doci_op *op, int kind);
 /* The operator's current sigma kernel (kinds as above). */
 int doci_op_kernel(const doci_op *op);
 /* The star split: a low / b high orbitals, main-pass and star-pass
- * workgroups, and the number of entries of the slot buffer. */
+ * workgroups, and the FP64 entries of the slot buffer (slots, then the
+ * packed K of the column star pass). */
 int doci_op_star_split(doci_op *op, unsigned *a, unsigned *b, unsigned *ngroups,
                        unsigned *nstargroups, size_t *zslots);
 /* The slab kernel's split: a low orbitals, b high orbitals, workgroups. */
@@ -137,8 +138,9 @@ int64_t doci_op_check_violations(doci_op *op);
 /* Tuning knobs.  Slab kernel: "xcd_runs" (0: groups dispatched in rank
  * order, 1: contiguous runs per XCD), "emax" (determinants staged per
  * workgroup, rebuilds the slab split), "gmax" (slabs per workgroup).  Star
- * kernel: "star_mfma" (1: star pass on the FP64 matrix cores, 0: vector
- * FMAs); and (each rebuilds the star tables; 0 restores the default):
+ * kernel: "star_pass" (0: column pass with wave-uniform K, the default;
+ * 1: per-star vector products; 2: per-star products on the matrix cores);
+ * and (each rebuilds the star tables; 0 restores the default):
  * "star_lds" (LDS bytes of a star-pass workgroup), "star_sgmax" (stars per
  * star-pass workgroup), "star_emax" (determinants staged per main-pass
  * workgroup). */
