@@ -206,7 +206,7 @@ static int build_shape(const dk_tables *dt, unsigned lds_budget, unsigned sgmax,
         if (maxS < MAIN_EMAX) maxS = MAIN_EMAX;
         if (ymax > 64) continue;
         if (star_lds_bytes(L, N, STAR_SGMAX, ymax, (unsigned)mx) <= STAR_LDS_BUDGET &&
-            8 * (size_t)b * b + 4 * (size_t)(b + 1) * (N + 2) + 8 * (size_t)maxS + 64 <=
+            8 * (size_t)b * b + 4 * (size_t)(b + 1) * (N + 2) + 16 * (size_t)maxS + 64 <=
                 MAIN_LDS_MAX) {
             /* the rest of the budget stages as many centres as fit */
             selems = (unsigned)((STAR_LDS_BUDGET - star_lds_bytes(L, N, STAR_SGMAX, ymax, 0)) / 16);
@@ -467,7 +467,7 @@ int dh_stars_set_pass(doci_op *op, int pass)
         const int rc = dh_stars_build(op, 0, 0, 0);
         if (rc != DOCI_OK) return rc;
     }
-    if (pass < 0 || pass > 2) return doci_fail(DOCI_E_ARG, "star_pass must be 0, 1 or 2");
+    if (pass < 0 || pass > 3) return doci_fail(DOCI_E_ARG, "star_pass must be 0 to 3");
     ((dh_stars *)op->stars)->pass = pass;
     return DOCI_OK;
 }

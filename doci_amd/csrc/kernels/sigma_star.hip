@@ -109,7 +109,7 @@ template <typename T>
 __host__ __device__ size_t main_lds(uint32_t L, uint32_t N, const dk_stars &st)
 {
     return a8(sizeof(double) * st.b * st.b) + a8(sizeof(uint32_t) * (st.b + 1) * (N + 2)) +
-           a8(sizeof(T) * st.emax);
+           a8(sizeof(T) * st.emax) + 2 * a8(sizeof(uint32_t) * st.emax);
 }
 
 // One lane per (star, member offset beta).  The lane loads the member slabs'
@@ -134,7 +134,7 @@ struct StarCtx {
 
 // Stages K, the binomials, the member tables and the centre slabs of star
 // group sg in LDS (zeroing the centre accumulators); ends with a barrier.
-template <typename T, bool kChecked, typename OK>
+template <typename T, bool kChecked, int NT = kBlock, typename OK>
 __device__ __forceinline__ StarCtx star_setup(const dk_tables &t, const dk_stars &st, uint32_t sg,
                                               const T *__restrict__ c, StarLds &p, OK &&ok)
 {
@@ -147,8 +147,8 @@ __device__ __forceinline__ StarCtx star_setup(const dk_tables &t, const dk_stars
     x.ztot = st.zoff[x.a + 1];
     const int L = x.L, N = x.N, a = x.a, b = x.b;
     const uint32_t n = x.n;
-    for (int i = threadIdx.x; i < L * L; i += kBlock) p.K[i] = t.K[i];
-    for (int i = threadIdx.x; i < (L + 1) * (N + 2); i += kBlock) p.bt[i] = t.binom[i];
+    for (int i = threadIdx.x; i < L * L; i += NT) p.K[i] = t.K[i];
+    for (int i = threadIdx.x; i < (L + 1) * (N + 2); i += NT) p.bt[i] = t.binom[i];
     const uint32_t first = st.sgroup[2 * sg], w = st.sgroup[2 * sg + 1];
     const int nst = (int)(w & 0xffffu), j = (int)(w >> 16);
     __syncthreads();
@@ -173,7 +173,7 @@ __device__ __forceinline__ StarCtx star_setup(const dk_tables &t, const dk_stars
     const uint64_t amask_all = a < 64 ? ((1ull << a) - 1) : ~0ull;
 
     // member tables: first rank, Z position of this star's slot, orbital y
-    for (int task = threadIdx.x; task < nst * ny; task += kBlock) {
+    for (int task = threadIdx.x; task < nst * ny; task += NT) {
         const int s = task / ny, yi = task - s * ny;
         const uint64_t rho = st.star_amask[first + s];
         uint64_t z = ~rho & amask_all;
@@ -187,7 +187,7 @@ __device__ __forceinline__ StarCtx star_setup(const dk_tables &t, const dk_stars
         p.ycol[s * ny + yi] = (uint8_t)y;
     }
     if (has_ctr)
-        for (int s = threadIdx.x; s < nst; s += kBlock) {
+        for (int s = threadIdx.x; s < nst; s += NT) {
             const uint64_t rho = st.star_amask[first + s];
             p.ctr_start[s] = slab_first(rho, mc, L, N, b, n, C);
             p.ctr_zb[s] = zc_sector + lex_a(rho, j, a, C) * (uint32_t)(nsc * Sc) + (uint32_t)(j * Sc);
@@ -195,23 +195,36 @@ __device__ __forceinline__ StarCtx star_setup(const dk_tables &t, const dk_stars
     __syncthreads();
     T *ctrv = (T *)p.ctr;
     if (has_ctr)
-        for (int e = threadIdx.x; e < nst * Sc; e += kBlock) {
-            const int s = e / Sc;
-            const uint32_t gi = p.ctr_start[s] + (uint32_t)(e - s * Sc);
-            ctrv[e] = ok(gi, n) ? c[gi] : (T)0;
-            p.acc[e] = 0.0;
+        for (int e0 = threadIdx.x; e0 < nst * Sc; e0 += 4 * NT) {
+            T v[4]; // four loads in flight per lane
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = e0 + u * NT;
+                v[u] = (T)0;
+                if (e < nst * Sc) {
+                    const int s = e / Sc;
+                    const uint32_t gi = p.ctr_start[s] + (uint32_t)(e - s * Sc);
+                    if (ok(gi, n)) v[u] = c[gi];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (e0 + u * NT < nst * Sc) {
+                    ctrv[e0 + u * NT] = v[u];
+                    p.acc[e0 + u * NT] = 0.0;
+                }
         }
     __syncthreads();
     return x;
 }
 
 // Writes the centre accumulators to their Z slots (after a barrier).
-template <typename T, typename OK>
+template <typename T, int NT = kBlock, typename OK>
 __device__ __forceinline__ void star_finish(const StarCtx &x, StarLds &p, T *__restrict__ Z, OK &&ok)
 {
     if (!x.has_ctr) return;
     __syncthreads();
-    for (int e = threadIdx.x; e < x.nst * x.Sc; e += kBlock) {
+    for (int e = threadIdx.x; e < x.nst * x.Sc; e += NT) {
         const int s = e / x.Sc;
         const uint32_t zi = p.ctr_zb[s] + (uint32_t)(e - s * x.Sc);
         if (ok(zi, x.ztot)) Z[zi] = (T)p.acc[e];
@@ -468,7 +481,9 @@ __global__ __launch_bounds__(kBlock) void k_star_mfma(dk_tables t, dk_stars st, 
 //   KP[AMAX * (AMAX + BMAX) + q * AMAX + x] = K_{x, a + q}
 __host__ __device__ constexpr int col_amax(int a) { return 8 * ((a + 7) / 8); }
 __host__ __device__ constexpr int col_bmax(int b) { return b <= 12 ? 12 : 16; }
-__host__ __device__ constexpr size_t col_kp_doubles(int am, int bm) { return (size_t)am * (am + bm) + (size_t)bm * am; }
+// (plus 16 doubles of zero padding: the column pass prefetches 16 entries of
+// whatever column comes next)
+__host__ __device__ constexpr size_t col_kp_doubles(int am, int bm) { return (size_t)am * (am + bm) + (size_t)bm * am + 16; }
 constexpr int kColAMax = 32; // larger A sets use the per-star kernel
 
 template <int AMAX, int BMAX>
@@ -488,7 +503,7 @@ __global__ __launch_bounds__(kBlock) void k_kpack(dk_tables t, int a, int b, dou
         }
         KP[i] = v;
     }
-    for (int i = threadIdx.x; i < BMAX * AMAX; i += kBlock) {
+    for (int i = threadIdx.x; i < BMAX * AMAX + 16; i += kBlock) {
         const int q = i / AMAX, xo = i - q * AMAX;
         KP[AMAX * R + i] = (q < b && xo < a) ? t.K[xo * L + a + q] : 0.0;
     }
@@ -506,30 +521,31 @@ __device__ __forceinline__ int opaque(int v, int tie)
 }
 
 // out[r] += k[r] * v for r < NR, k in global memory read through scalar
-// registers 8 entries at a time: kbuf holds the first 8 entries of k on
-// entry, and on exit the first 8 of kn (the next call's k), which are loaded
-// while the last chunk's FMAs run.  The scheduling barriers keep one chunk
-// in flight, so at most 16 scalar registers hold K.
+// registers kKC entries at a time: kbuf holds the first kKC entries of k on
+// entry, and on exit the first kKC of kn (the next call's k), which are
+// loaded while the last chunk's FMAs run.  The scheduling barriers keep one
+// chunk in flight beside the one in use.
+constexpr int kKC = 8;
 template <int NR, int NOUT>
 __device__ __forceinline__ void col_fma(const double *k, const double *kn, double v, double (&out)[NOUT],
-                                        double (&kbuf)[8])
+                                        double (&kbuf)[kKC])
 {
-    constexpr int NCH = (NR + 7) / 8;
+    constexpr int NCH = (NR + kKC - 1) / kKC;
 #pragma unroll
     for (int ch = 0; ch < NCH; ++ch) {
-        double knx[8];
+        double knx[kKC];
         const bool last = ch + 1 == NCH;
-        const double *src = last ? kn : k + 8 * (ch + 1);
-        const int cnt = last ? 8 : (NR - 8 * (ch + 1) < 8 ? NR - 8 * (ch + 1) : 8);
+        const double *src = last ? kn : k + kKC * (ch + 1);
+        const int cnt = last ? kKC : (NR - kKC * (ch + 1) < kKC ? NR - kKC * (ch + 1) : kKC);
 #pragma unroll
-        for (int i = 0; i < 8; ++i) knx[i] = i < cnt ? src[i] : 0.0;
+        for (int i = 0; i < kKC; ++i) knx[i] = i < cnt ? src[i] : 0.0;
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int i = 0; i < 8; ++i)
-            if (8 * ch + i < NR) out[8 * ch + i] = __builtin_fma(kbuf[i], v, out[8 * ch + i]);
+        for (int i = 0; i < kKC; ++i)
+            if (kKC * ch + i < NR) out[kKC * ch + i] = __builtin_fma(kbuf[i], v, out[kKC * ch + i]);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-        for (int i = 0; i < 8; ++i) kbuf[i] = knx[i];
+        for (int i = 0; i < kKC; ++i) kbuf[i] = knx[i];
     }
 }
 
@@ -619,11 +635,11 @@ __global__ __launch_bounds__(kBlock, 3) void k_star_col(dk_tables t, dk_stars st
         for (int r = 0; r < R; ++r) out[r] = 0.0;
         const double *kctr = kp + AMAX * R; // centre columns
         const uint64_t ym = ~common & amask; // the wave's orbitals y
-        double kbuf[8];
+        double kbuf[kKC];
         {
             const double *k0 = ym ? kp + (__ffsll((unsigned long long)ym) - 1) * R : kctr;
 #pragma unroll
-            for (int i = 0; i < 8; ++i) kbuf[i] = k0[i];
+            for (int i = 0; i < kKC; ++i) kbuf[i] = k0[i];
         }
 #pragma unroll
         for (int y = 0; y < AMAX; ++y) {
@@ -682,6 +698,157 @@ __global__ __launch_bounds__(kBlock, 3) void k_star_col(dk_tables t, dk_stars st
     star_finish<T>(x, p, Z, ok);
 }
 
+// FP32 star pass on the matrix cores.  The column formulation of
+// k_star_col (products over all of A, K independent of the star) with
+// v_mfma_f32_16x16x4f32 tiles: a wave takes 16 columns at a time; the B
+// operands are the columns' member values X_y (k index y = 4 kb + lane / 16)
+// and centre values; the A operands are K in FP32, the same for every column,
+// so they stay in registers for the whole kernel.  Each 4-term product step
+// and the sum over the steps accumulate in FP32 (the slot is stored in FP32);
+// the main pass sums a determinant's slots in FP64.  Lane l holds A[l % 16]
+// [l / 16], B[l / 16][l % 16] and, in result register r, D[4 (l / 16) + r]
+// [l % 16] (layout checked on gfx950).  YK: 4-orbital steps over A (a <= 4 YK).
+constexpr int kBlockM = 512;
+typedef float dk_f4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ int m32_row(int l, int r) { return 4 * (l >> 4) + r; }
+
+template <bool kChecked, int YK>
+__global__ __launch_bounds__(kBlockM) void k_star_m32(dk_tables t, dk_stars st, const float *__restrict__ c,
+                                                      float *__restrict__ Z, uint32_t *bad,
+                                                      const dk_sigma_io *io)
+{
+    constexpr int XT = (4 * YK + 15) / 16; // row tiles of members
+    constexpr int kW = kBlockM / 64;
+    extern __shared__ double lds_raw[];
+    const uint32_t sg = blockIdx.x;
+    if (sg >= st.nsgroup) return;
+    if (io) {
+        if (!io->active) return;
+        c = (const float *)io->c;
+    }
+    StarLds p;
+    star_lds<float>(t.L, t.N, st, &p, (char *)lds_raw);
+    auto ok = [&](uint32_t idx, uint32_t lim) -> bool {
+        if (!kChecked) return true;
+        if (idx < lim) return true;
+        atomicAdd(bad, 1u);
+        return false;
+    };
+    const StarCtx x = star_setup<float, kChecked, kBlockM>(t, st, sg, c, p, ok);
+    const int L = (int)t.L, a = (int)st.a, b = (int)st.b;
+    const int ny = __builtin_amdgcn_readfirstlane(x.ny);
+    const int Sm = __builtin_amdgcn_readfirstlane(x.Sm);
+    const int Sc = __builtin_amdgcn_readfirstlane(x.Sc);
+    const int nst = __builtin_amdgcn_readfirstlane(x.nst);
+    const bool has_ctr = __builtin_amdgcn_readfirstlane((int)x.has_ctr) != 0;
+    if (a > 4 * YK || b > 16) { // uniform: the launcher picks 4 YK >= a
+        if (kChecked && threadIdx.x == 0) atomicAdd(bad, 1u);
+        return;
+    }
+    const int lane = (int)(threadIdx.x & 63), li = lane & 15, lk = lane >> 4;
+    // A operands: K in FP32 (zero outside A x A, B x A, A x B and on the diagonal)
+    float ax[XT][YK], axc[XT][4], at[YK];
+#pragma unroll
+    for (int xt = 0; xt < XT; ++xt) {
+        const int xo = 16 * xt + li;
+#pragma unroll
+        for (int kb = 0; kb < YK; ++kb) {
+            const int y = 4 * kb + lk;
+            ax[xt][kb] = (xo < a && y < a && xo != y) ? (float)p.K[xo * L + y] : 0.f;
+        }
+#pragma unroll
+        for (int tb = 0; tb < 4; ++tb) {
+            const int tt = 4 * tb + lk;
+            axc[xt][tb] = (xo < a && tt < b) ? (float)p.K[xo * L + a + tt] : 0.f;
+        }
+    }
+#pragma unroll
+    for (int kb = 0; kb < YK; ++kb) {
+        const int y = 4 * kb + lk;
+        at[kb] = (li < b && y < a) ? (float)p.K[(a + li) * L + y] : 0.f;
+    }
+    const uint32_t first = st.sgroup[2 * sg];
+    const int ncol = nst * Sm, nunit = (ncol + 15) / 16;
+    const uint64_t amask = a < 64 ? ((1ull << a) - 1) : ~0ull;
+    const float *ctrv = (const float *)p.ctr;
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    for (int u = wave; u < nunit; u += kW) {
+        const int col = 16 * u + li;
+        const bool live = col < ncol;
+        const int s = live ? col / Sm : 0;
+        const int beta = live ? col - s * Sm : 0;
+        const uint64_t rho = live ? st.star_amask[first + s] : amask;
+        const uint64_t rhox = rho | ~amask; // orbitals without a member
+        const uint32_t *ms = p.mem_start + s * ny;
+        const uint32_t *mz = p.mem_zb + s * ny;
+        // B operands: member values of this lane's column at y = 4 kb + lane / 16
+        float xb[YK];
+#pragma unroll
+        for (int kb = 0; kb < YK; ++kb) {
+            const int y = 4 * kb + lk;
+            xb[kb] = 0.f;
+            if (live && !((rhox >> y) & 1)) {
+                const int yi = y - __popcll(rho & ((1ull << y) - 1));
+                const uint32_t gi = ms[yi] + (uint32_t)beta;
+                if (ok(gi, x.n)) xb[kb] = c[gi];
+            }
+        }
+        // centre values at lexB(beta + t), t = 4 tb + lane / 16
+        float cb[4];
+#pragma unroll
+        for (int tb = 0; tb < 4; ++tb) {
+            const int tt = 4 * tb + lk;
+            cb[tb] = 0.f;
+            if (has_ctr && live && tt < b) {
+                const uint32_t off = x.ins[(size_t)tt * Sm + beta];
+                if (off != 0xffffu && ok(off, (uint32_t)Sc)) cb[tb] = ctrv[s * Sc + off];
+            }
+        }
+        dk_f4 dx[XT];
+#pragma unroll
+        for (int xt = 0; xt < XT; ++xt) {
+            dx[xt] = dk_f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kb = 0; kb < YK; ++kb)
+                dx[xt] = __builtin_amdgcn_mfma_f32_16x16x4f32(ax[xt][kb], xb[kb], dx[xt], 0, 0, 0);
+#pragma unroll
+            for (int tb = 0; tb < 4; ++tb)
+                dx[xt] = __builtin_amdgcn_mfma_f32_16x16x4f32(axc[xt][tb], cb[tb], dx[xt], 0, 0, 0);
+        }
+        // member slots: this lane's column, rows m32_row(lane, r) of each tile
+#pragma unroll
+        for (int xt = 0; xt < XT; ++xt)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int xo = 16 * xt + m32_row(lane, r);
+                if (live && !((rhox >> xo) & 1)) {
+                    const int xi = xo - __popcll(rho & ((1ull << xo) - 1));
+                    const uint32_t zi = mz[xi] + (uint32_t)beta;
+                    if (ok(zi, x.ztot)) Z[zi] = dx[xt][r];
+                }
+            }
+        // the centre's hops from B orbital t into A, into its accumulator
+        if (has_ctr) {
+            dk_f4 dt = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int kb = 0; kb < YK; ++kb)
+                dt = __builtin_amdgcn_mfma_f32_16x16x4f32(at[kb], xb[kb], dt, 0, 0, 0);
+            double *cacc = p.acc + s * Sc;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int tt = m32_row(lane, r);
+                if (live && tt < b) {
+                    const uint32_t off = x.ins[(size_t)tt * Sm + beta];
+                    if (off != 0xffffu && ok(off, (uint32_t)Sc))
+                        __hip_atomic_fetch_add(&cacc[off], (double)dt[r], __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+            }
+        }
+    }
+    star_finish<float, kBlockM>(x, p, Z, ok);
+}
+
 template <typename T, bool kChecked>
 __global__ __launch_bounds__(kBlock) void k_star_main(dk_tables t, dk_stars st,
                                                       const double *__restrict__ D, double shift,
@@ -710,6 +877,9 @@ __global__ __launch_bounds__(kBlock) void k_star_main(dk_tables t, dk_stars st,
     double *Kbb = lds_raw; // Kbb[o * b + t] = K between B orbitals o and t
     uint32_t *bt = (uint32_t *)((char *)lds_raw + a8(sizeof(double) * b * b));
     T *own = (T *)((char *)bt + a8(sizeof(uint32_t) * (b + 1) * (N + 2)));
+    // the group's slab starts and slot bases (a group has at most emax slabs)
+    uint32_t *gstart = (uint32_t *)((char *)own + a8(sizeof(T) * st.emax));
+    uint32_t *gzb = gstart + a8(sizeof(uint32_t) * st.emax) / sizeof(uint32_t);
     for (int i = threadIdx.x; i < b * b; i += kBlock) {
         const int o = i / b, tt = i - o * b;
         Kbb[i] = t.K[(a + o) * L + a + tt];
@@ -718,36 +888,73 @@ __global__ __launch_bounds__(kBlock) void k_star_main(dk_tables t, dk_stars st,
     const uint32_t first = st.group[2 * g], w = st.group[2 * g + 1];
     const int nsl = (int)(w & 0xffffu), m = (int)(w >> 16), k = N - m;
     __syncthreads();
-    const dk::Binom C{bt, N + 2};
-    const int S = (int)C(b, m), nel = nsl * S, ns = nslot_of(k, m, a);
-    for (int e = threadIdx.x; e < nel; e += kBlock) {
-        const int s = e / S, bi = e - s * S;
-        const uint32_t gi = st.slab_start[first + s] + (uint32_t)bi;
-        own[e] = ok(gi, n) ? c[gi] : (T)0;
+    for (int i = threadIdx.x; i < nsl; i += kBlock) {
+        gstart[i] = st.slab_start[first + i];
+        gzb[i] = st.slab_zbase[first + i];
     }
     __syncthreads();
+    const dk::Binom C{bt, N + 2};
+    const int S = (int)C(b, m), nel = nsl * S, ns = nslot_of(k, m, a);
+    // stage the own slabs: four loads in flight per lane
+    for (int e0 = threadIdx.x; e0 < nel; e0 += 4 * kBlock) {
+        T v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = e0 + u * kBlock;
+            v[u] = (T)0;
+            if (e < nel) {
+                const int s = e / S, bi = e - s * S;
+                const uint32_t gi = gstart[s] + (uint32_t)bi;
+                if (ok(gi, n)) v[u] = c[gi];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (e0 + u * kBlock < nel) own[e0 + u * kBlock] = v[u];
+    }
+    __syncthreads();
+    const int nh = m * (b - m);
+    constexpr int kZS = 16; // slots loaded together (more are summed by a tail loop)
+    constexpr int kHB = 16; // hop-table entries loaded together
     for (int e = threadIdx.x; e < nel; e += kBlock) {
         const int s = e / S, bi = e - s * S;
-        const uint32_t I = st.slab_start[first + s] + (uint32_t)bi;
+        const uint32_t I = gstart[s] + (uint32_t)bi;
         if (!ok(I, n)) continue;
         const T *os = own + s * S;
-        double acc = (D[I] - shift) * (double)os[bi];
-        // hops inside B from the hop table (coalesced across lanes)
-        const uint32_t *hp = st.hop + st.hop_off[m] + bi;
-        const int nh = m * (b - m);
-#pragma unroll 4
-        for (int h = 0; h < nh; ++h) {
-            const uint32_t hv = hp[(size_t)h * S];
-            const uint32_t J = hv & 0xffffu, kx = hv >> 16;
-            if (ok(J, (uint32_t)S) && ok(kx, (uint32_t)(b * b))) acc += Kbb[kx] * (double)os[J];
+        // the slots and the diagonal first (HBM), all in flight together
+        const uint32_t zb = gzb[s] + (uint32_t)bi;
+        T zv[kZS];
+#pragma unroll
+        for (int q = 0; q < kZS; ++q) {
+            zv[q] = (T)0;
+            if (q < ns) {
+                const uint32_t zi = zb + (uint32_t)(q * S);
+                if (ok(zi, ztot)) zv[q] = Z[zi];
+            }
         }
-        const uint32_t zb = st.slab_zbase[first + s] + (uint32_t)bi;
+        const double dI = D[I];
+        // hops inside B from the hop table (coalesced across lanes), kHB at a time
+        const uint32_t *hp = st.hop + st.hop_off[m] + bi;
+        double acc = 0;
+        for (int h0 = 0; h0 < nh; h0 += kHB) {
+            uint32_t hv[kHB];
+#pragma unroll
+            for (int i = 0; i < kHB; ++i) hv[i] = h0 + i < nh ? hp[(size_t)(h0 + i) * S] : 0u;
+#pragma unroll
+            for (int i = 0; i < kHB; ++i) {
+                const uint32_t J = hv[i] & 0xffffu, kx = hv[i] >> 16;
+                if (h0 + i < nh && ok(J, (uint32_t)S) && ok(kx, (uint32_t)(b * b)))
+                    acc += Kbb[kx] * (double)os[J];
+            }
+        }
         double zs = 0;
-        for (int q = 0; q < ns; ++q) {
+#pragma unroll
+        for (int q = 0; q < kZS; ++q) zs += (double)zv[q];
+        for (int q = kZS; q < ns; ++q) {
             const uint32_t zi = zb + (uint32_t)(q * S);
             if (ok(zi, ztot)) zs += (double)Z[zi];
         }
-        sigma[I] = (T)(acc + zs);
+        sigma[I] = (T)(((dI - shift) * (double)os[bi] + acc) + zs);
     }
 }
 
@@ -759,7 +966,31 @@ int launch(const dk_tables *t, const dk_stars *st, const double *D, double shift
     if (st->b > 16 || st->a > 64) return (int)hipErrorInvalidValue;
     const int a = (int)st->a, b = (int)st->b, N = (int)t->N;
     int j = 0;
-    if (st->pass == 0 && a >= 1 && a <= kColAMax) {
+    if (sizeof(T) == 4 && st->pass == 3 && a >= 1 && a <= 32) {
+        // FP32: the column pass on the FP32 matrix cores, one launch
+        const int yk = (a + 3) / 4;
+        const dim3 grid(st->nsgroup);
+        switch (yk) {
+#define DK_M32_CASE(Y)                                                                             \
+    case Y:                                                                                        \
+        hipLaunchKernelGGL((k_star_m32<kChecked, Y>), grid, dim3(kBlockM), ls, s, *t, *st,        \
+                           (const float *)c, (float *)Z, bad, io);                                 \
+        break;
+            DK_M32_CASE(1)
+            DK_M32_CASE(2)
+            DK_M32_CASE(3)
+            DK_M32_CASE(4)
+            DK_M32_CASE(5)
+            DK_M32_CASE(6)
+            DK_M32_CASE(7)
+            DK_M32_CASE(8)
+#undef DK_M32_CASE
+        }
+        const int e = (int)hipGetLastError();
+        if (e) return e;
+        j = N; // the other star passes below are skipped
+    }
+    if ((st->pass == 0 || st->pass == 3) && j < N && a >= 1 && a <= kColAMax) {
         // one launch for every star group: K packed behind the slots, then the column pass
         double *kp = (double *)((char *)Z + sizeof(double) * (size_t)st->zoff[a + 1]);
         const int am = col_amax(a), bm = col_bmax(b);

@@ -138,8 +138,10 @@ int64_t doci_op_check_violations(doci_op *op);
 /* Tuning knobs.  Slab kernel: "xcd_runs" (0: groups dispatched in rank
  * order, 1: contiguous runs per XCD), "emax" (determinants staged per
  * workgroup, rebuilds the slab split), "gmax" (slabs per workgroup).  Star
- * kernel: "star_pass" (0: column pass with wave-uniform K, the default;
- * 1: per-star vector products; 2: per-star products on the matrix cores);
+ * kernel: "star_pass" (0, the default: column pass with wave-uniform K; 1:
+ * per-star vector products; 2: per-star products on the FP64 matrix cores;
+ * 3: the column pass on the FP32 matrix cores for FP32 vectors, 4-term FP32
+ * partial sums);
  * and (each rebuilds the star tables; 0 restores the default):
  * "star_lds" (LDS bytes of a star-pass workgroup), "star_sgmax" (stars per
  * star-pass workgroup), "star_emax" (determinants staged per main-pass
