@@ -84,16 +84,65 @@ int finalize(const double *red, int nblocks, int nv, double *out, hipStream_t s)
     return (int)hipGetLastError();
 }
 
+// Each lane streams VW consecutive elements per step with 16-byte (or, when
+// the per-element registers of an M-vector kernel would not fit, narrower)
+// loads; the last n % VW elements take a scalar tail loop.  All vectors are
+// 256-byte aligned (device allocations and subspace slots).
+template <typename T, int VW> struct Vec {
+    typedef T type __attribute__((ext_vector_type(VW)));
+};
+template <typename T> struct Vec<T, 1> {
+    typedef T type;
+};
+template <typename T, int VW>
+__device__ __forceinline__ double el(const typename Vec<T, VW>::type &v, int e)
+{
+    if constexpr (VW == 1) return (double)v;
+    else return (double)v[e];
+}
+template <typename T, int VW>
+__device__ __forceinline__ void set_el(typename Vec<T, VW>::type &v, int e, T x)
+{
+    if constexpr (VW == 1) v = x;
+    else v[e] = x;
+}
+template <typename T, int VW>
+__device__ __forceinline__ typename Vec<T, VW>::type ldv(const void *p, int64_t q)
+{
+    return ((const typename Vec<T, VW>::type *)p)[q];
+}
+// elements per lane step: 16 bytes, fewer when M values per element are kept
+template <typename T, int M>
+constexpr int vw_for()
+{
+    constexpr int w = 16 / (int)sizeof(T);
+    return M <= 8 ? w : (M <= 16 ? (w >= 2 ? w / 2 : 1) : 1);
+}
+
 // ---- multidot: out[0] = <y,y>, out[1+j] = <X_j, y> -------------------------
 template <typename T, int M>
 __global__ __launch_bounds__(kBlock) void k_multidot(Ptrs X, int m, const T *__restrict__ y,
                                                      int64_t n, double *__restrict__ red)
 {
+    constexpr int VW = 16 / (int)sizeof(T);
     double acc[M + 1];
 #pragma unroll
     for (int k = 0; k <= M; ++k) acc[k] = 0;
-    const int64_t stride = (int64_t)gridDim.x * kBlock;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+    const int64_t stride = (int64_t)gridDim.x * kBlock, nq = n / VW;
+    const int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    for (int64_t q = tid; q < nq; q += stride) {
+        const auto yv = ldv<T, VW>(y, q);
+#pragma unroll
+        for (int e = 0; e < VW; ++e) acc[0] += el<T, VW>(yv, e) * el<T, VW>(yv, e);
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+            if (j < m) {
+                const auto xv = ldv<T, VW>(X.p[j], q);
+#pragma unroll
+                for (int e = 0; e < VW; ++e) acc[1 + j] += el<T, VW>(xv, e) * el<T, VW>(yv, e);
+            }
+    }
+    for (int64_t i = nq * VW + tid; i < n; i += stride) {
         const double yi = (double)y[i];
         acc[0] += yi * yi;
 #pragma unroll
@@ -104,93 +153,159 @@ __global__ __launch_bounds__(kBlock) void k_multidot(Ptrs X, int m, const T *__r
 }
 
 // ---- ortho: y -= sum g_j X_j; out[0] = <y,y>, out[1+j] = <X_j,y> -----------
+template <typename T, int M, int VW>
+__device__ __forceinline__ void ortho_step(const Ptrs &X, int m, const Coef &g, T *__restrict__ y,
+                                           int64_t q, double (&acc)[M + 1])
+{
+    typedef typename Vec<T, VW>::type VT;
+    VT xv[M];
+    VT yv = ((VT *)y)[q];
+    double yd[VW];
+#pragma unroll
+    for (int e = 0; e < VW; ++e) yd[e] = el<T, VW>(yv, e);
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+        if (j < m) {
+            xv[j] = ldv<T, VW>(X.p[j], q);
+#pragma unroll
+            for (int e = 0; e < VW; ++e) yd[e] -= g.g[j] * el<T, VW>(xv[j], e);
+        }
+#pragma unroll
+    for (int e = 0; e < VW; ++e) {
+        set_el<T, VW>(yv, e, (T)yd[e]);
+        yd[e] = el<T, VW>(yv, e);
+        acc[0] += yd[e] * yd[e];
+    }
+    ((VT *)y)[q] = yv;
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+        if (j < m)
+#pragma unroll
+            for (int e = 0; e < VW; ++e) acc[1 + j] += el<T, VW>(xv[j], e) * yd[e];
+}
+
 template <typename T, int M>
 __global__ __launch_bounds__(kBlock) void k_ortho(Ptrs X, int m, Coef g, T *__restrict__ y,
                                                   int64_t n, double *__restrict__ red)
 {
+    constexpr int VW = vw_for<T, M>();
     double acc[M + 1];
 #pragma unroll
     for (int k = 0; k <= M; ++k) acc[k] = 0;
-    const int64_t stride = (int64_t)gridDim.x * kBlock;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-        double v[M];
-        double yv = (double)y[i];
-#pragma unroll
-        for (int j = 0; j < M; ++j)
-            if (j < m) {
-                v[j] = ld<T>(X.p[j], i);
-                yv -= g.g[j] * v[j];
-            }
-        const T yq = (T)yv;
-        y[i] = yq;
-        const double yd = (double)yq;
-        acc[0] += yd * yd;
-#pragma unroll
-        for (int j = 0; j < M; ++j)
-            if (j < m) acc[1 + j] += v[j] * yd;
-    }
+    const int64_t stride = (int64_t)gridDim.x * kBlock, nq = n / VW;
+    const int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    for (int64_t q = tid; q < nq; q += stride) ortho_step<T, M, VW>(X, m, g, y, q, acc);
+    for (int64_t i = nq * VW + tid; i < n; i += stride) ortho_step<T, M, 1>(X, m, g, y, i, acc);
     block_partials<M + 1>(acc, m + 1, red);
 }
 
 // ---- Davidson residual (src/davidson.c:195-227 fused) ---------------------
 // out[0] = <r,r>, out[1] = <t,t>, out[2+j] = <V_j,t>
+template <typename T, int M, int VW>
+__device__ __forceinline__ void residual_step(const Ptrs &V, const Ptrs &W, int m, const Coef &a,
+                                              double theta, double shift,
+                                              const double *__restrict__ D, T *__restrict__ t,
+                                              T *__restrict__ xo, int64_t q, double (&acc)[M + 2])
+{
+    typedef typename Vec<T, VW>::type VT;
+    VT vv[M];
+    double x[VW], w[VW];
+#pragma unroll
+    for (int e = 0; e < VW; ++e) x[e] = w[e] = 0;
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+        if (j < m) {
+            vv[j] = ldv<T, VW>(V.p[j], q);
+            const VT wv = ldv<T, VW>(W.p[j], q);
+#pragma unroll
+            for (int e = 0; e < VW; ++e) {
+                x[e] += a.g[j] * el<T, VW>(vv[j], e);
+                w[e] += a.g[j] * el<T, VW>(wv, e);
+            }
+        }
+    VT tv, xv;
+    double td[VW];
+#pragma unroll
+    for (int e = 0; e < VW; ++e) {
+        const double r = w[e] - (theta - shift) * x[e];
+        const double d = fabs(D[q * VW + e] - theta);
+        const double tval = d > 1e-12 ? r / d : r / 1e-12;
+        set_el<T, VW>(tv, e, (T)tval);
+        set_el<T, VW>(xv, e, (T)x[e]);
+        td[e] = el<T, VW>(tv, e);
+        acc[0] += r * r;
+        acc[1] += td[e] * td[e];
+    }
+    ((VT *)t)[q] = tv;
+    if (xo) ((VT *)xo)[q] = xv;
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+        if (j < m)
+#pragma unroll
+            for (int e = 0; e < VW; ++e) acc[2 + j] += el<T, VW>(vv[j], e) * td[e];
+}
+
 template <typename T, int M>
 __global__ __launch_bounds__(kBlock) void k_residual(Ptrs V, Ptrs W, int m, Coef a, double theta,
                                                      double shift, const double *__restrict__ D,
                                                      T *__restrict__ t, T *__restrict__ xo,
                                                      int64_t n, double *__restrict__ red)
 {
+    constexpr int VW = vw_for<T, M>();
     double acc[M + 2];
 #pragma unroll
     for (int k = 0; k < M + 2; ++k) acc[k] = 0;
-    const int64_t stride = (int64_t)gridDim.x * kBlock;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-        double v[M];
-        double x = 0, w = 0;
-#pragma unroll
-        for (int j = 0; j < M; ++j)
-            if (j < m) {
-                v[j] = ld<T>(V.p[j], i);
-                x += a.g[j] * v[j];
-                w += a.g[j] * ld<T>(W.p[j], i);
-            }
-        const double r = w - (theta - shift) * x;
-        const double d = fabs(D[i] - theta);
-        const double tv = d > 1e-12 ? r / d : r / 1e-12;
-        const T tq = (T)tv;
-        t[i] = tq;
-        if (xo) xo[i] = (T)x;
-        const double td = (double)tq;
-        acc[0] += r * r;
-        acc[1] += td * td;
-#pragma unroll
-        for (int j = 0; j < M; ++j)
-            if (j < m) acc[2 + j] += v[j] * td;
-    }
+    const int64_t stride = (int64_t)gridDim.x * kBlock, nq = n / VW;
+    const int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    for (int64_t q = tid; q < nq; q += stride)
+        residual_step<T, M, VW>(V, W, m, a, theta, shift, D, t, xo, q, acc);
+    for (int64_t i = nq * VW + tid; i < n; i += stride)
+        residual_step<T, M, 1>(V, W, m, a, theta, shift, D, t, xo, i, acc);
     block_partials<M + 2>(acc, m + 2, red);
+}
+
+template <typename T, typename To, int M, int VW>
+__device__ __forceinline__ void lincomb_step(const Ptrs &X, int m, const Coef &g, To *__restrict__ out,
+                                             int64_t q)
+{
+    double s[VW];
+#pragma unroll
+    for (int e = 0; e < VW; ++e) s[e] = 0;
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+        if (j < m) {
+            const auto xv = ldv<T, VW>(X.p[j], q);
+#pragma unroll
+            for (int e = 0; e < VW; ++e) s[e] += g.g[j] * el<T, VW>(xv, e);
+        }
+#pragma unroll
+    for (int e = 0; e < VW; ++e) out[q * VW + e] = (To)s[e];
 }
 
 template <typename T, typename To, int M>
 __global__ __launch_bounds__(kBlock) void k_lincomb(Ptrs X, int m, Coef g, To *__restrict__ out,
                                                     int64_t n)
 {
-    const int64_t stride = (int64_t)gridDim.x * kBlock;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-        double s = 0;
-#pragma unroll
-        for (int j = 0; j < M; ++j)
-            if (j < m) s += g.g[j] * ld<T>(X.p[j], i);
-        out[i] = (To)s;
-    }
+    constexpr int VW = 16 / (int)sizeof(T);
+    const int64_t stride = (int64_t)gridDim.x * kBlock, nq = n / VW;
+    const int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    for (int64_t q = tid; q < nq; q += stride) lincomb_step<T, To, M, VW>(X, m, g, out, q);
+    for (int64_t i = nq * VW + tid; i < n; i += stride) lincomb_step<T, To, M, 1>(X, m, g, out, i);
 }
 
 template <typename Ti, typename To>
 __global__ __launch_bounds__(kBlock) void k_convert(const Ti *__restrict__ in, To *__restrict__ out,
                                                     int64_t n, double scale)
 {
-    const int64_t stride = (int64_t)gridDim.x * kBlock;
-    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride)
-        out[i] = (To)(scale * (double)in[i]);
+    constexpr int VW = 16 / (int)sizeof(Ti);
+    const int64_t stride = (int64_t)gridDim.x * kBlock, nq = n / VW;
+    const int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+    for (int64_t q = tid; q < nq; q += stride) {
+        const auto v = ldv<Ti, VW>(in, q);
+#pragma unroll
+        for (int e = 0; e < VW; ++e) out[q * VW + e] = (To)(scale * el<Ti, VW>(v, e));
+    }
+    for (int64_t i = nq * VW + tid; i < n; i += stride) out[i] = (To)(scale * (double)in[i]);
 }
 
 // ---- CG (src/CG.c) ---------------------------------------------------------
