@@ -240,9 +240,14 @@ static int uses_star(uint32_t n) { return n >= DOCI_STAR_MIN_N; }
  * (few partial sums to reduce, long coalesced streams per workgroup). */
 #define MAX_TILES_PER_INSTANCE 2048u
 
+/* Rows per tile of an instance: the batch's tile size, or more so that the
+ * instance has at most MAX_TILES_PER_INSTANCE tiles.  Always a multiple of
+ * 512: the streaming kernels read whole 16-byte vectors from every tile's
+ * start. */
 static uint32_t rows_for(uint32_t n, uint32_t base)
 {
-    const uint32_t cap = (n + MAX_TILES_PER_INSTANCE - 1) / MAX_TILES_PER_INSTANCE;
+    uint32_t cap = (n + MAX_TILES_PER_INSTANCE - 1) / MAX_TILES_PER_INSTANCE;
+    cap = (cap + 511) / 512 * 512;
     return base > cap ? base : cap;
 }
 
@@ -273,6 +278,7 @@ int doci_batch_create(const doci_batch_opts *o_in, doci_batch **out)
         return doci_fail(DOCI_E_HIP, "no HIP device available");
     doci_batch *b = calloc(1, sizeof *b);
     if (!b) return doci_fail(DOCI_E_NOMEM, "batch");
+    if (o.tile_rows > 0) o.tile_rows = (o.tile_rows + 511) / 512 * 512; /* see rows_for */
     b->o = o;
     b->cap = o.max_instances;
     b->t0 = dh_now();

@@ -73,6 +73,171 @@ __device__ __forceinline__ void tile_partials(double (&acc)[NV], int nv, double 
 }
 
 // ---- streaming steps -------------------------------------------------------
+// Each lane streams VW consecutive rows per step with one vector load per
+// vector (16 bytes, narrower when M per-row values are kept); tiles start at
+// multiples of 512 rows and slots are 256-byte aligned, so only an
+// instance's last tile has a scalar tail.
+template <typename T, int VW> struct Vec {
+    typedef T type __attribute__((ext_vector_type(VW)));
+};
+template <typename T> struct Vec<T, 1> {
+    typedef T type;
+};
+template <typename T, int VW>
+__device__ __forceinline__ double el(const typename Vec<T, VW>::type &v, int e)
+{
+    if constexpr (VW == 1) return (double)v;
+    else return (double)v[e];
+}
+template <typename T, int VW>
+__device__ __forceinline__ void set_el(typename Vec<T, VW>::type &v, int e, T x)
+{
+    if constexpr (VW == 1) v = x;
+    else v[e] = x;
+}
+template <typename T, int VW>
+__device__ __forceinline__ typename Vec<T, VW>::type ldv(const T *p, uint32_t q)
+{
+    return ((const typename Vec<T, VW>::type *)p)[q];
+}
+template <typename T, int VW>
+__device__ __forceinline__ void stv(T *p, uint32_t q, const typename Vec<T, VW>::type &v)
+{
+    ((typename Vec<T, VW>::type *)p)[q] = v;
+}
+template <typename T, int M>
+constexpr int vw_for()
+{
+    constexpr int w = 16 / (int)sizeof(T);
+    return M <= 8 ? w : (M <= 16 ? (w >= 2 ? w / 2 : 1) : 1);
+}
+
+// Rows [r0, r1) of one tile: steps of VW rows (chunk index q = row / VW),
+// then the scalar tail.
+#define DK_ROWS(STEP)                                                                              \
+    do {                                                                                           \
+        const uint32_t rv = r0 + (r1 - r0) / VW * VW;                                              \
+        for (uint32_t q = r0 / VW + threadIdx.x; q < rv / VW; q += kBlock) STEP(VW, q);            \
+        for (uint32_t q = rv + threadIdx.x; q < r1; q += kBlock) STEP(1, q);                       \
+    } while (0)
+
+template <typename T, int M, int W>
+__device__ __forceinline__ void ortho_rows(const T *const (&V)[M], int m, const double (&g)[M], T *t,
+                                           uint32_t q, double (&acc)[M + 1])
+{
+    typedef typename Vec<T, W>::type VT;
+    VT xv[M];
+    VT tv = ldv<T, W>(t, q);
+    double td[W];
+#pragma unroll
+    for (int e = 0; e < W; ++e) td[e] = el<T, W>(tv, e);
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+        if (j < m) {
+            xv[j] = ldv<T, W>(V[j], q);
+#pragma unroll
+            for (int e = 0; e < W; ++e) td[e] -= g[j] * el<T, W>(xv[j], e);
+        }
+#pragma unroll
+    for (int e = 0; e < W; ++e) {
+        set_el<T, W>(tv, e, (T)td[e]);
+        td[e] = el<T, W>(tv, e);
+        acc[0] += td[e] * td[e];
+    }
+    if (m > 0) stv<T, W>(t, q, tv);
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+        if (j < m)
+#pragma unroll
+            for (int e = 0; e < W; ++e) acc[1 + j] += el<T, W>(xv[j], e) * td[e];
+}
+
+template <typename T, int M, int W>
+__device__ __forceinline__ void scol_rows(const T *const (&Vp)[M + 1], int m, const T *Wm, uint32_t q,
+                                          double (&acc)[M + 1])
+{
+    const auto wv = ldv<T, W>(Wm, q);
+#pragma unroll
+    for (int j = 0; j <= M; ++j)
+        if (j <= m) {
+            const auto vv = ldv<T, W>(Vp[j], q);
+#pragma unroll
+            for (int e = 0; e < W; ++e) acc[j] += el<T, W>(vv, e) * el<T, W>(wv, e);
+        }
+}
+
+template <typename T, int M, int W>
+__device__ __forceinline__ void deflate_rows(const T *const (&V)[M], const T *const (&Wv)[M], int m,
+                                             const double (&c)[M], T *dv, T *dw, uint32_t q)
+{
+    typedef typename Vec<T, W>::type VT;
+    double a[W], b[W];
+#pragma unroll
+    for (int e = 0; e < W; ++e) a[e] = b[e] = 0;
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+        if (j < m) {
+            const VT vv = ldv<T, W>(V[j], q), wv = ldv<T, W>(Wv[j], q);
+#pragma unroll
+            for (int e = 0; e < W; ++e) {
+                a[e] += c[j] * el<T, W>(vv, e);
+                b[e] += c[j] * el<T, W>(wv, e);
+            }
+        }
+    VT ov, ow;
+#pragma unroll
+    for (int e = 0; e < W; ++e) {
+        set_el<T, W>(ov, e, (T)a[e]);
+        set_el<T, W>(ow, e, (T)b[e]);
+    }
+    stv<T, W>(dv, q, ov);
+    stv<T, W>(dw, q, ow);
+}
+
+template <typename T, int M, int W>
+__device__ __forceinline__ void resid_rows(const T *const (&V)[M], const T *const (&Wv)[M], int m,
+                                           const double (&yc)[M], double theta, double E,
+                                           const double *D, T *t, double *x, uint32_t q,
+                                           double (&acc)[M + 2])
+{
+    typedef typename Vec<T, W>::type VT;
+    VT vv[M];
+    double xs[W], ws[W];
+#pragma unroll
+    for (int e = 0; e < W; ++e) xs[e] = ws[e] = 0;
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+        if (j < m) {
+            vv[j] = ldv<T, W>(V[j], q);
+            const VT wv = ldv<T, W>(Wv[j], q);
+#pragma unroll
+            for (int e = 0; e < W; ++e) {
+                xs[e] += yc[j] * el<T, W>(vv[j], e);
+                ws[e] += yc[j] * el<T, W>(wv, e);
+            }
+        }
+    VT tv;
+    double td[W];
+#pragma unroll
+    for (int e = 0; e < W; ++e) {
+        const uint32_t i = q * W + e;
+        const double r = ws[e] - theta * xs[e];
+        const double d = fabs(D[i] - E);
+        const double tval = d > 1e-12 ? r / d : r / 1e-12;
+        set_el<T, W>(tv, e, (T)tval);
+        td[e] = el<T, W>(tv, e);
+        x[i] = xs[e];
+        acc[0] += r * r;
+        acc[1] += td[e] * td[e];
+    }
+    stv<T, W>(t, q, tv);
+#pragma unroll
+    for (int j = 0; j < M; ++j)
+        if (j < m)
+#pragma unroll
+            for (int e = 0; e < W; ++e) acc[2 + j] += el<T, W>(vv[j], e) * td[e];
+}
+
 template <typename T, int M, int OP>
 __global__ __launch_bounds__(kBlock) void k_bstream(const dk_binst *__restrict__ insts,
                                                     dk_bstate *__restrict__ states,
@@ -96,6 +261,7 @@ __global__ __launch_bounds__(kBlock) void k_bstream(const dk_binst *__restrict__
     for (int j = 0; j < M; ++j) V[j] = j < m ? slot_ptr<T>(in, st.vslot[j]) : nullptr;
 
     if constexpr (OP == DK_BOP_ORTHO || OP == DK_BOP_ORTHO2) {
+        constexpr int VW = vw_for<T, M>();
         T *t = slot_ptr<T>(in, st.tslot);
         double g[M];
 #pragma unroll
@@ -103,25 +269,12 @@ __global__ __launch_bounds__(kBlock) void k_bstream(const dk_binst *__restrict__
         double acc[M + 1];
 #pragma unroll
         for (int k = 0; k <= M; ++k) acc[k] = 0;
-        for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
-            double v[M];
-            double tv = (double)t[i];
-#pragma unroll
-            for (int j = 0; j < M; ++j)
-                if (j < m) {
-                    v[j] = (double)V[j][i];
-                    tv -= g[j] * v[j];
-                }
-            const T tq = (T)tv;
-            if (m > 0) t[i] = tq;
-            const double td = (double)tq;
-            acc[0] += td * td;
-#pragma unroll
-            for (int j = 0; j < M; ++j)
-                if (j < m) acc[1 + j] += v[j] * td;
-        }
+#define DK_STEP(W_, q_) ortho_rows<T, M, W_>(V, m, g, t, q_, acc)
+        DK_ROWS(DK_STEP);
+#undef DK_STEP
         tile_partials<M + 1>(acc, m + 1, out);
     } else if constexpr (OP == DK_BOP_SCOL) {
+        constexpr int VW = 16 / (int)sizeof(T);
         // column m: V_0..V_m (V_m is t itself) against W_m, the sigma just computed
         const T *Vp[M + 1];
 #pragma unroll
@@ -130,14 +283,12 @@ __global__ __launch_bounds__(kBlock) void k_bstream(const dk_binst *__restrict__
         double acc[M + 1];
 #pragma unroll
         for (int k = 0; k <= M; ++k) acc[k] = 0;
-        for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
-            const double w = (double)Wm[i];
-#pragma unroll
-            for (int j = 0; j <= M; ++j)
-                if (j <= m) acc[j] += (double)Vp[j][i] * w;
-        }
+#define DK_STEP(W_, q_) scol_rows<T, M, W_>(Vp, m, Wm, q_, acc)
+        DK_ROWS(DK_STEP);
+#undef DK_STEP
         tile_partials<M + 1>(acc, m + 1, out);
     } else if constexpr (OP == DK_BOP_DEFLATE) {
+        constexpr int VW = vw_for<T, M>();
         const int keep = st.keep;
         const T *W[M];
 #pragma unroll
@@ -148,19 +299,12 @@ __global__ __launch_bounds__(kBlock) void k_bstream(const dk_binst *__restrict__
             for (int j = 0; j < M; ++j) c[j] = j < m ? st.defl[k][j] : 0.0;
             T *dv = slot_ptr<T>(in, st.dv[k]);
             T *dw = slot_ptr<T>(in, st.dw[k]);
-            for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
-                double a = 0, b = 0;
-#pragma unroll
-                for (int j = 0; j < M; ++j)
-                    if (j < m) {
-                        a += c[j] * (double)V[j][i];
-                        b += c[j] * (double)W[j][i];
-                    }
-                dv[i] = (T)a;
-                dw[i] = (T)b;
-            }
+#define DK_STEP(W_, q_) deflate_rows<T, M, W_>(V, W, m, c, dv, dw, q_)
+            DK_ROWS(DK_STEP);
+#undef DK_STEP
         }
     } else { // DK_BOP_RESID
+        constexpr int VW = vw_for<T, M>();
         T *t = slot_ptr<T>(in, st.tslot);
         const double E = st.theta + st.shift, theta = st.theta;
         double yc[M];
@@ -173,32 +317,13 @@ __global__ __launch_bounds__(kBlock) void k_bstream(const dk_binst *__restrict__
         double acc[M + 2];
 #pragma unroll
         for (int k = 0; k < M + 2; ++k) acc[k] = 0;
-        for (uint32_t i = r0 + threadIdx.x; i < r1; i += kBlock) {
-            double v[M];
-            double x = 0, w = 0;
-#pragma unroll
-            for (int j = 0; j < M; ++j)
-                if (j < m) {
-                    v[j] = (double)V[j][i];
-                    x += yc[j] * v[j];
-                    w += yc[j] * (double)W[j][i];
-                }
-            const double r = w - theta * x;
-            const double d = fabs(in.D[i] - E);
-            const double tv = d > 1e-12 ? r / d : r / 1e-12;
-            const T tq = (T)tv;
-            t[i] = tq;
-            in.x[i] = x;
-            const double td = (double)tq;
-            acc[0] += r * r;
-            acc[1] += td * td;
-#pragma unroll
-            for (int j = 0; j < M; ++j)
-                if (j < m) acc[2 + j] += v[j] * td;
-        }
+#define DK_STEP(W_, q_) resid_rows<T, M, W_>(V, W, m, yc, theta, E, in.D, t, in.x, q_, acc)
+        DK_ROWS(DK_STEP);
+#undef DK_STEP
         tile_partials<M + 2>(acc, m + 2, out);
     }
 }
+#undef DK_ROWS
 
 // ---- batched lane sigma ----------------------------------------------------
 template <typename T>

@@ -427,13 +427,27 @@ __global__ __launch_bounds__(kBlock) void k_argmin_part(const double *__restrict
     }
 }
 
-__global__ void k_argmin_fin(const double *__restrict__ red, int nb, int64_t *__restrict__ idx)
+// (value, index) pairs are totally ordered, so the result does not depend on
+// the order of the merges.
+__global__ __launch_bounds__(kBlock) void k_argmin_fin(const double *__restrict__ red, int nb,
+                                                       int64_t *__restrict__ idx)
 {
-    if (threadIdx.x == 0) {
-        double v = INFINITY, ix = 1e300;
-        for (int b = 0; b < nb; ++b) amin_merge(v, ix, red[2 * b], red[2 * b + 1]);
-        *idx = (int64_t)ix;
+    __shared__ double sv[kBlock], si[kBlock];
+    double v = INFINITY, ix = 1e300;
+    for (int b = threadIdx.x; b < nb; b += kBlock) amin_merge(v, ix, red[2 * b], red[2 * b + 1]);
+    sv[threadIdx.x] = v;
+    si[threadIdx.x] = ix;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            double a = sv[threadIdx.x], b = si[threadIdx.x];
+            amin_merge(a, b, sv[threadIdx.x + w], si[threadIdx.x + w]);
+            sv[threadIdx.x] = a;
+            si[threadIdx.x] = b;
+        }
+        __syncthreads();
     }
+    if (threadIdx.x == 0) *idx = (int64_t)si[0];
 }
 
 } // namespace
@@ -624,6 +638,6 @@ extern "C" int dk_argmin(const double *D, int64_t n, double *red, int64_t *idx, 
 {
     const int nb = grid_for(n);
     hipLaunchKernelGGL(k_argmin_part, dim3(nb), dim3(kBlock), 0, s, D, n, red);
-    hipLaunchKernelGGL(k_argmin_fin, dim3(1), dim3(64), 0, s, red, nb, idx);
+    hipLaunchKernelGGL(k_argmin_fin, dim3(1), dim3(kBlock), 0, s, red, nb, idx);
     return (int)hipGetLastError();
 }

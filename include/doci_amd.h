@@ -246,7 +246,8 @@ typedef struct {
     int max_vec;           /* 30 (at most 30) */
     double switch_tol;     /* mixed instances: FP32 phase until this residual (1e-5) */
     int use_graphs;        /* capture each iteration's launches in a HIP graph (1) */
-    int tile_rows;         /* rows per workgroup of the batched launches; 0 = autotune */
+    int tile_rows;         /* rows per workgroup of the batched launches, rounded up to a
+                              multiple of 512; 0 = autotune */
     size_t mem_budget;     /* bytes of device memory for running instances (0 = 90% free) */
     int max_instances;     /* capacity (1024) */
     int keep_vectors;      /* 1: keep a finished instance's eigenvector until

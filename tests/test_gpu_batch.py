@@ -179,12 +179,14 @@ def test_batch_large_instances_use_star_kernel():
     cases = [(23, 10, 71), (24, 9, 72), (14, 5, 73)]
     refs = [_single(*c) for c in cases]
     assert da.binom(10, 23) >= 1 << 20 and da.binom(9, 24) >= 1 << 20
-    for prec in ("fp64", "mixed"):
-        with da.Batch(tile_rows=2048) as b:
+    # tile rows 512: the two large instances need more rows per tile than that
+    # (at most 2048 tiles each), which the engine rounds to a multiple of 512
+    for prec, rows in (("fp64", 2048), ("mixed", 2048), ("fp64", 512), ("mixed", 512)):
+        with da.Batch(tile_rows=rows) as b:
             ids = [b.add_synthetic(*c, precision=prec) for c in cases]
             b.run()
             for i, c, ref in zip(ids, cases, refs):
                 r, x = b.result(i, want_x=True)
-                assert r.status == 2 and r.residual <= 1e-8, (prec, c)
-                assert abs(r.energy - ref.energy) <= 1e-9, (prec, c, r.energy - ref.energy)
-                assert abs(abs(x @ ref.x) - 1) <= 1e-8, (prec, c)
+                assert r.status == 2 and r.residual <= 1e-8, (prec, rows, c)
+                assert abs(r.energy - ref.energy) <= 1e-9, (prec, rows, c, r.energy - ref.energy)
+                assert abs(abs(x @ ref.x) - 1) <= 1e-8, (prec, rows, c)
