@@ -21,7 +21,10 @@
  * W entries are small and FP32 rounding of W stays below the switch
  * tolerance); once the residual drops below switch_tol the solve restarts in
  * FP64 from the current Ritz vector and finishes with FP64 residuals and
- * Rayleigh quotients.
+ * Rayleigh quotients.  The restart keeps the FP32 phase's last correction
+ * (its preconditioned residual, as an FP64 vector): the first FP64 iteration
+ * computes the FP64 sigma of the Ritz vector and then orthogonalises that
+ * correction, instead of recomputing the residual of the same vector.
  */
 #include <math.h>
 #include <stdio.h>
@@ -132,9 +135,14 @@ static int add_sigma_time(dav_run *r, int prec)
 }
 
 /* One Davidson phase at storage precision `prec`; stops at residual <=
- * stop_tol.  d_x: FP64 guess in, FP64 Ritz vector out. */
+ * stop_tol.  d_x: FP64 guess in, FP64 Ritz vector out.  t_in (FP64, may be
+ * NULL): the correction of the guess, already known (the preconditioned
+ * residual of the previous phase), used instead of recomputing it, with
+ * res_in its residual norm.  t_out (FP64, may be NULL): receives the
+ * correction that the next iteration would have used. */
 static int dav_phase(dav_run *r, int prec, double shift, double stop_tol, double *d_x,
-                     double *energy, double *res_out)
+                     const double *t_in, double res_in, double *t_out, double *energy,
+                     double *res_out)
 {
     dh_ctx *c = r->c;
     const doci_solve_opts *o = r->o;
@@ -253,16 +261,28 @@ static int dav_phase(dav_run *r, int prec, double shift, double stop_tol, double
             g[j] = s[j] * Y[j];
         }
         tslot = take_slot(&w);
-        DK(dk_residual(m, Vp, Wp, g, theta, shift, c->d_D, w.Vs[tslot], NULL, n, prec, c->d_red,
-                       c->d_out, c->stream));
-        CK(dh_fetch(c, m + 2, red));
-        res = sqrt(red[0]);
-        tt = red[1];
-        for (int j = 0; j < m; ++j) a[j] = red[2 + j];
+        if (t_in && m == 1 && *r->its == start_its + 1) {
+            /* the guess's correction is given: its overlaps for the next
+             * orthogonalisation (same subspace as recomputing it) */
+            DK(dk_convert(t_in, 0, w.Vs[tslot], prec, n, 1.0, c->stream));
+            DK(dk_multidot(m, Vp, w.Vs[tslot], n, prec, c->d_red, c->d_out, c->stream));
+            CK(dh_fetch(c, m + 1, red));
+            res = res_in;
+            tt = red[0];
+            for (int j = 0; j < m; ++j) a[j] = red[1 + j];
+        } else {
+            DK(dk_residual(m, Vp, Wp, g, theta, shift, c->d_D, w.Vs[tslot], NULL, n, prec, c->d_red,
+                           c->d_out, c->stream));
+            CK(dh_fetch(c, m + 2, red));
+            res = sqrt(red[0]);
+            tt = red[1];
+            for (int j = 0; j < m; ++j) a[j] = red[2 + j];
+        }
         *energy = theta;
         if (prec) r->info->iterations_fp32++;
         if (o->trace) printf("%d\t%d\t%e\t%f\n", *r->its, 0, res, theta);
     }
+    if (t_out && m > 0 && res <= stop_tol) DK(dk_convert(w.Vs[tslot], prec, t_out, 0, n, 1.0, c->stream));
     /* Ritz vector x = V y (davidson.c:205) */
     if (m > 0) {
         for (int j = 0; j < m; ++j) {
@@ -300,7 +320,9 @@ int dh_davidson(dh_ctx *c, const doci_solve_opts *o, double *d_x, double *energy
         return doci_fail(DOCI_E_ARG, "need 1 <= DAVIDSON_KEEP < DAVIDSON_MAX_VEC <= %d", DK_MAX_VEC);
     const double t0 = dh_now();
     int its = 0, rc;
-    double res = 0;
+    double res = 0, res32 = 0;
+    double *d_t = NULL; /* FP64 correction handed from the FP32 phase to the FP64 phase */
+    int t_ok = 0;
     dav_run r = {c, o, info, &its, NULL, NULL};
     HIPCHK(hipEventCreate(&r.ev0));
     HIPCHK(hipEventCreate(&r.ev1));
@@ -318,15 +340,24 @@ int dh_davidson(dh_ctx *c, const doci_solve_opts *o, double *d_x, double *energy
             goto out;
         }
         const double sw = o->switch_tol > o->tol ? o->switch_tol : o->tol;
-        rc = dav_phase(&r, 1, shift, sw, d_x, energy, &res);
+        /* the FP32 phase's last correction seeds the FP64 phase: its first
+         * iteration computes the FP64 sigma of the Ritz vector and goes on
+         * with that correction instead of a second residual of the same x */
+        if (hipMalloc((void **)&d_t, sizeof(double) * (size_t)c->n) != hipSuccess) d_t = NULL;
+        rc = dav_phase(&r, 1, shift, sw, d_x, NULL, 0.0, d_t, energy, &res);
         if (rc) goto out;
+        res32 = res;
+        t_ok = d_t && res <= sw; /* the phase wrote its last correction */
     }
     if (res > o->tol || o->precision != 1 || !c->fp32_ok) {
-        if (its < o->max_its || its == 0) rc = dav_phase(&r, 0, 0.0, o->tol, d_x, energy, &res);
+        const int seeded = t_ok && res32 > o->tol;
+        if (its < o->max_its || its == 0)
+            rc = dav_phase(&r, 0, 0.0, o->tol, d_x, seeded ? d_t : NULL, res32, NULL, energy, &res);
         if (rc) goto out;
     }
     rc = its >= o->max_its;
 out:
+    if (d_t) hipFree(d_t);
     hipEventDestroy(r.ev0);
     hipEventDestroy(r.ev1);
     info->iterations = its;
