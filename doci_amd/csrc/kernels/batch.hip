@@ -109,7 +109,7 @@ template <typename T, int M>
 constexpr int vw_for()
 {
     constexpr int w = 16 / (int)sizeof(T);
-    return M <= 8 ? w : (M <= 16 ? (w >= 2 ? w / 2 : 1) : 1);
+    return M <= 16 ? w : (M <= 32 ? (w >= 2 ? w / 2 : 1) : (w >= 4 ? w / 4 : 1));
 }
 
 // Rows [r0, r1) of one tile: steps of VW rows (chunk index q = row / VW),

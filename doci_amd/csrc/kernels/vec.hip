@@ -111,12 +111,12 @@ __device__ __forceinline__ typename Vec<T, VW>::type ldv(const void *p, int64_t 
 {
     return ((const typename Vec<T, VW>::type *)p)[q];
 }
-// elements per lane step: 16 bytes, fewer when M values per element are kept
+// elements per lane step: 16 bytes, fewer when 32 or more values per element are kept
 template <typename T, int M>
 constexpr int vw_for()
 {
     constexpr int w = 16 / (int)sizeof(T);
-    return M <= 8 ? w : (M <= 16 ? (w >= 2 ? w / 2 : 1) : 1);
+    return M <= 16 ? w : (M <= 32 ? (w >= 2 ? w / 2 : 1) : (w >= 4 ? w / 4 : 1));
 }
 
 // ---- multidot: out[0] = <y,y>, out[1+j] = <X_j, y> -------------------------
