@@ -99,7 +99,8 @@ typedef struct {
      * bmask[bmask_off[m] + r] = beta;
      * hop[hop_off[m] + h * C(b,m) + r] = J | (o * b + t) << 16 for the h-th
      *   hop beta - o + t (o in beta ascending, then t not in beta ascending),
-     *   J = its lexB rank;
+     *   J = its lexB rank; rows h from m(b-m) up to the next multiple of 8
+     *   are padding, r | (b * b) << 16 (index b * b names a zero K entry);
      * ins[ins_off[m] + t * C(b,m) + r] = lexB rank of beta + t among the
      *   (m+1)-pair masks, 0xffff when t is in beta (m < b). */
     const uint16_t *bmask;

@@ -100,6 +100,10 @@ static uint32_t lex_rank(const int *idx, unsigned m, unsigned b)
 }
 
 /* The B-mask, hop and insert tables of dk_stars, uploaded to s->dtab. */
+/* Rows of the hop table of m-pair masks: the m(b-m) hops, padded to a
+ * multiple of 8 so that the main pass loads them 8 at a time unguarded. */
+static size_t hop_rows(unsigned m, unsigned b) { return ((size_t)m * (b - m) + 7) / 8 * 8; }
+
 static int build_b_tables(star_shape *s, unsigned b)
 {
     size_t nmask = 0, nhop = 0, nins = 0;
@@ -109,7 +113,7 @@ static int build_b_tables(star_shape *s, unsigned b)
         s->d.hop_off[m] = (uint32_t)nhop;
         s->d.ins_off[m] = (uint32_t)nins;
         nmask += S;
-        nhop += S * m * (b - m);
+        nhop += S * hop_rows(m, b);
         if (m < b) nins += S * b;
     }
     const size_t bytes_m = sizeof(uint16_t) * nmask, bytes_h = sizeof(uint32_t) * nhop,
@@ -148,6 +152,8 @@ static int build_b_tables(star_shape *s, unsigned b)
                     ++hh;
                 }
             }
+            for (; hh < hop_rows(m, b); ++hh) /* padding: a hop onto itself with K = 0 */
+                hop[s->d.hop_off[m] + hh * S + r] = (uint32_t)r | (uint32_t)(b * b) << 16;
             if (m < b)
                 for (unsigned t = 0; t < b; ++t) {
                     uint16_t v = 0xffff;

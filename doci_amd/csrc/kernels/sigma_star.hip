@@ -108,7 +108,7 @@ __host__ __device__ size_t star_lds(uint32_t L, uint32_t N, const dk_stars &st, 
 template <typename T>
 __host__ __device__ size_t main_lds(uint32_t L, uint32_t N, const dk_stars &st)
 {
-    return a8(sizeof(double) * st.b * st.b) + a8(sizeof(uint32_t) * (st.b + 1) * (N + 2)) +
+    return a8(sizeof(double) * (st.b * st.b + 1)) + a8(sizeof(uint32_t) * (st.b + 1) * (N + 2)) +
            a8(sizeof(T) * st.emax) + 2 * a8(sizeof(uint32_t) * st.emax);
 }
 
@@ -874,15 +874,15 @@ __global__ __launch_bounds__(kBlock) void k_star_main(dk_tables t, dk_stars st,
         atomicAdd(bad, 1u);
         return false;
     };
-    double *Kbb = lds_raw; // Kbb[o * b + t] = K between B orbitals o and t
-    uint32_t *bt = (uint32_t *)((char *)lds_raw + a8(sizeof(double) * b * b));
+    double *Kbb = lds_raw; // Kbb[o * b + t] = K between B orbitals o and t; Kbb[b * b] = 0
+    uint32_t *bt = (uint32_t *)((char *)lds_raw + a8(sizeof(double) * (b * b + 1)));
     T *own = (T *)((char *)bt + a8(sizeof(uint32_t) * (b + 1) * (N + 2)));
     // the group's slab starts and slot bases (a group has at most emax slabs)
     uint32_t *gstart = (uint32_t *)((char *)own + a8(sizeof(T) * st.emax));
     uint32_t *gzb = gstart + a8(sizeof(uint32_t) * st.emax) / sizeof(uint32_t);
-    for (int i = threadIdx.x; i < b * b; i += kBlock) {
+    for (int i = threadIdx.x; i <= b * b; i += kBlock) {
         const int o = i / b, tt = i - o * b;
-        Kbb[i] = t.K[(a + o) * L + a + tt];
+        Kbb[i] = i < b * b ? t.K[(a + o) * L + a + tt] : 0.0;
     }
     for (int i = threadIdx.x; i < (b + 1) * (N + 2); i += kBlock) bt[i] = t.binom[i];
     const uint32_t first = st.group[2 * g], w = st.group[2 * g + 1];
@@ -913,9 +913,12 @@ __global__ __launch_bounds__(kBlock) void k_star_main(dk_tables t, dk_stars st,
             if (e0 + u * kBlock < nel) own[e0 + u * kBlock] = v[u];
     }
     __syncthreads();
-    const int nh = m * (b - m);
     constexpr int kZS = 16; // slots loaded together (more are summed by a tail loop)
-    constexpr int kHB = 16; // hop-table entries loaded together
+    constexpr int kHB = 8;  // hop-table rows loaded together (rows are padded to a multiple)
+    const int nhp = (m * (b - m) + kHB - 1) / kHB * kHB;
+    // the hop table of m-pair masks, nhp rows of S entries (loads past it return 0)
+    const __amdgpu_buffer_rsrc_t hrsrc = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)(st.hop + st.hop_off[m]), (short)0, nhp * S * 4, 0x00020000);
     for (int e = threadIdx.x; e < nel; e += kBlock) {
         const int s = e / S, bi = e - s * S;
         const uint32_t I = gstart[s] + (uint32_t)bi;
@@ -933,18 +936,20 @@ __global__ __launch_bounds__(kBlock) void k_star_main(dk_tables t, dk_stars st,
             }
         }
         const double dI = D[I];
-        // hops inside B from the hop table (coalesced across lanes), kHB at a time
-        const uint32_t *hp = st.hop + st.hop_off[m] + bi;
+        // hops inside B from the hop table (coalesced across lanes), 8 rows at
+        // a time through a buffer resource: the row offset is a scalar, the
+        // lane's column offset computed once, padding rows add 0
+        const uint32_t voff = (uint32_t)bi * 4u;
         double acc = 0;
-        for (int h0 = 0; h0 < nh; h0 += kHB) {
+        for (int h0 = 0; h0 < nhp; h0 += kHB) {
             uint32_t hv[kHB];
 #pragma unroll
-            for (int i = 0; i < kHB; ++i) hv[i] = h0 + i < nh ? hp[(size_t)(h0 + i) * S] : 0u;
+            for (int i = 0; i < kHB; ++i)
+                hv[i] = __builtin_amdgcn_raw_buffer_load_b32(hrsrc, (int)voff, (h0 + i) * S * 4, 0);
 #pragma unroll
             for (int i = 0; i < kHB; ++i) {
                 const uint32_t J = hv[i] & 0xffffu, kx = hv[i] >> 16;
-                if (h0 + i < nh && ok(J, (uint32_t)S) && ok(kx, (uint32_t)(b * b)))
-                    acc += Kbb[kx] * (double)os[J];
+                if (ok(J, (uint32_t)S) && ok(kx, (uint32_t)(b * b + 1))) acc += Kbb[kx] * (double)os[J];
             }
         }
         double zs = 0;
