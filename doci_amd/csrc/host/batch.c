@@ -276,6 +276,9 @@ int doci_batch_create(const doci_batch_opts *o_in, doci_batch **out)
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
         return doci_fail(DOCI_E_HIP, "no HIP device available");
+    /* the batch budgets and reserves free device memory: hand the buffers
+     * cached for single solves back to the driver first */
+    dh_dcache_trim();
     doci_batch *b = calloc(1, sizeof *b);
     if (!b) return doci_fail(DOCI_E_NOMEM, "batch");
     if (o.tile_rows > 0) o.tile_rows = (o.tile_rows + 511) / 512 * 512; /* see rows_for */

@@ -25,7 +25,7 @@ int dh_ray_q(dh_ctx *c, const doci_solve_opts *o, int precond, double *d_x, doub
     hipEvent_t e0 = NULL, e1 = NULL;
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
-    hipError_t he = hipMalloc((void **)&blk, vb * (precond ? 5 : 4));
+    hipError_t he = dh_dmalloc((void **)&blk, vb * (precond ? 5 : 4));
     if (he != hipSuccess) {
         hipEventDestroy(e0);
         hipEventDestroy(e1);
@@ -121,7 +121,8 @@ done:
 #undef CK
 #undef SIGMA
 #undef SIGMA_TIME
-    hipFree(blk);
+    hipStreamSynchronize(c->stream);
+    dh_dfree(blk);
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     return rc;

@@ -47,7 +47,7 @@ typedef struct {
 
 static void ws_free(dav_ws *w)
 {
-    if (w->block && !w->pooled) hipFree(w->block);
+    if (w->block && !w->pooled) dh_dfree(w->block);
     free(w->Vs);
     free(w->Ws);
     free(w->basis);
@@ -76,17 +76,17 @@ static int ws_alloc(dav_ws *w, dh_ctx *c, int64_t n, int prec, int nslot)
         w->block = *c->pool; /* reuse the operator's subspace memory */
         w->pooled = 1;
     } else if (c->pool) {
-        if (*c->pool) hipFree(*c->pool);
+        dh_dfree(*c->pool);
         *c->pool = NULL;
         *c->pool_bytes = 0;
-        e = hipMalloc(&w->block, need);
+        e = dh_dmalloc(&w->block, need);
         if (e == hipSuccess) {
             *c->pool = w->block;
             *c->pool_bytes = need;
             w->pooled = 1;
         }
     } else {
-        e = hipMalloc(&w->block, need);
+        e = dh_dmalloc(&w->block, need);
     }
     if (e != hipSuccess) {
         w->block = NULL;
@@ -343,7 +343,7 @@ int dh_davidson(dh_ctx *c, const doci_solve_opts *o, double *d_x, double *energy
         /* the FP32 phase's last correction seeds the FP64 phase: its first
          * iteration computes the FP64 sigma of the Ritz vector and goes on
          * with that correction instead of a second residual of the same x */
-        if (hipMalloc((void **)&d_t, sizeof(double) * (size_t)c->n) != hipSuccess) d_t = NULL;
+        if (dh_dmalloc((void **)&d_t, sizeof(double) * (size_t)c->n) != hipSuccess) d_t = NULL;
         rc = dav_phase(&r, 1, shift, sw, d_x, NULL, 0.0, d_t, energy, &res);
         if (rc) goto out;
         res32 = res;
@@ -357,7 +357,10 @@ int dh_davidson(dh_ctx *c, const doci_solve_opts *o, double *d_x, double *energy
     }
     rc = its >= o->max_its;
 out:
-    if (d_t) hipFree(d_t);
+    if (d_t) {
+        hipStreamSynchronize(c->stream);
+        dh_dfree(d_t);
+    }
     hipEventDestroy(r.ev0);
     hipEventDestroy(r.ev1);
     info->iterations = its;

@@ -13,6 +13,13 @@
  * decomposition tables to build); from it on, the star kernel. */
 #define DOCI_STAR_MIN_N (1u << 20)
 
+/* Process-wide cache of device buffers (devmem.c): released buffers are
+ * kept for the next request they fit.  Release only what no stream still
+ * uses. */
+hipError_t dh_dmalloc(void **p, size_t bytes);
+void dh_dfree(void *p);
+void dh_dcache_trim(void);
+
 struct doci_op {
     uint64_t magic;
     unsigned L, N;

@@ -39,14 +39,16 @@ int dh_is_op(const void *p) { return p && ((const doci_op *)p)->magic == DOCI_OP
 void doci_op_free(doci_op *op)
 {
     if (!op) return;
-    if (op->d_tables) hipFree(op->d_tables);
-    if (op->d_D) hipFree(op->d_D);
-    if (op->d_red) hipFree(op->d_red);
-    if (op->d_out) hipFree(op->d_out);
-    if (op->d_idx) hipFree(op->d_idx);
-    if (op->d_io[0]) hipFree(op->d_io[0]);
-    if (op->d_io[1]) hipFree(op->d_io[1]);
-    if (op->pool) hipFree(op->pool);
+    /* buffers go back to the process-wide cache: nothing may still use them */
+    if (op->stream) hipStreamSynchronize(op->stream);
+    dh_dfree(op->d_tables);
+    dh_dfree(op->d_D);
+    dh_dfree(op->d_red);
+    dh_dfree(op->d_out);
+    dh_dfree(op->d_idx);
+    dh_dfree(op->d_io[0]);
+    dh_dfree(op->d_io[1]);
+    dh_dfree(op->pool);
     dh_slabs_free(op);
     dh_stars_free(op);
     if (op->h_out) hipHostFree(op->h_out);
@@ -189,7 +191,7 @@ static int op_device_setup(doci_op *op)
     } while (0)
     TRY(hipStreamCreateWithFlags(&op->stream, hipStreamNonBlocking));
     const size_t tb = sizeof(double) * (3 * l2 + l) + sizeof(uint32_t) * (l + 1) * (size_t)N2;
-    TRY(hipMalloc(&op->d_tables, tb));
+    TRY(dh_dmalloc(&op->d_tables, tb));
     {
         char *b = (char *)op->d_tables;
         double *dK = (double *)b, *dJ = dK + l2, *dX = dJ + l2, *dh = dX + l2;
@@ -210,10 +212,10 @@ static int op_device_setup(doci_op *op)
         op->dt.h = dh;
         op->dt.binom = db;
     }
-    TRY(hipMalloc((void **)&op->d_D, sizeof(double) * n));
-    TRY(hipMalloc((void **)&op->d_red, sizeof(double) * dk_red_scratch_doubles(DK_MAX_VEC + 8)));
-    TRY(hipMalloc((void **)&op->d_out, sizeof(double) * (DK_MAX_VEC + 8)));
-    TRY(hipMalloc((void **)&op->d_idx, sizeof(int64_t)));
+    TRY(dh_dmalloc((void **)&op->d_D, sizeof(double) * n));
+    TRY(dh_dmalloc((void **)&op->d_red, sizeof(double) * dk_red_scratch_doubles(DK_MAX_VEC + 8)));
+    TRY(dh_dmalloc((void **)&op->d_out, sizeof(double) * (DK_MAX_VEC + 8)));
+    TRY(dh_dmalloc((void **)&op->d_idx, sizeof(int64_t)));
     TRY(hipHostMalloc((void **)&op->h_out, sizeof(double) * (DK_MAX_VEC + 8), 0));
     TRY((hipError_t)dk_diagonal(&op->dt, op->d_D, op->stream));
     TRY(hipStreamSynchronize(op->stream));
@@ -327,7 +329,7 @@ int doci_op_slab_split(doci_op *op, unsigned *a, unsigned *b, unsigned *ngroups)
 static int ensure_io(doci_op *op)
 {
     for (int k = 0; k < 2; ++k)
-        if (!op->d_io[k]) HIPCHK(hipMalloc(&op->d_io[k], sizeof(double) * op->n));
+        if (!op->d_io[k]) HIPCHK(dh_dmalloc(&op->d_io[k], sizeof(double) * op->n));
     return DOCI_OK;
 }
 

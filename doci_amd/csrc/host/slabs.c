@@ -36,8 +36,8 @@ void dh_slabs_free(doci_op *op)
 {
     dh_slabs *s = op->slabs;
     if (!s) return;
-    if (s->dmem) hipFree(s->dmem);
-    if (s->d_bad) hipFree(s->d_bad);
+    dh_dfree(s->dmem);
+    dh_dfree(s->d_bad);
     free(s);
     op->slabs = NULL;
 }
@@ -144,7 +144,7 @@ int dh_slabs_build(doci_op *op, unsigned emax_req, unsigned gmax, int xcd_runs)
     }
     const size_t b_amask = sizeof(uint64_t) * nslab, b_start = sizeof(uint32_t) * nslab,
                  b_grp = sizeof(uint32_t) * 2 * ng;
-    hipError_t e = hipMalloc(&s->dmem, b_amask + b_start + b_grp);
+    hipError_t e = dh_dmalloc(&s->dmem, b_amask + b_start + b_grp);
     if (e != hipSuccess) {
         free(s);
         rc = doci_fail(DOCI_E_NOMEM, "slab tables on device: %s", hipGetErrorString(e));
@@ -155,7 +155,7 @@ int dh_slabs_build(doci_op *op, unsigned emax_req, unsigned gmax, int xcd_runs)
     if (e == hipSuccess) e = hipMemcpy(dm + b_amask, start, b_start, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(dm + b_amask + b_start, gw, b_grp, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
-        hipFree(s->dmem);
+        dh_dfree(s->dmem);
         free(s);
         rc = doci_fail(DOCI_E_HIP, "slab table upload: %s", hipGetErrorString(e));
         goto out;
@@ -176,15 +176,15 @@ int dh_slabs_build(doci_op *op, unsigned emax_req, unsigned gmax, int xcd_runs)
     s->lds[1] = dk_sigma_slab_lds(&op->dt, &s->d, 1);
     if (s->lds[0] > SLAB_LDS_BUDGET) {
         rc = doci_fail(DOCI_E_ARG, "slab kernel LDS %zu exceeds budget", s->lds[0]);
-        hipFree(s->dmem);
+        dh_dfree(s->dmem);
         free(s);
         goto out;
     }
-    e = hipMalloc((void **)&s->d_bad, sizeof(uint32_t));
+    e = dh_dmalloc((void **)&s->d_bad, sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(s->d_bad, 0, sizeof(uint32_t));
     if (e != hipSuccess) {
         rc = doci_fail(DOCI_E_HIP, "slab counter: %s", hipGetErrorString(e));
-        hipFree(s->dmem);
+        dh_dfree(s->dmem);
         free(s);
         goto out;
     }
@@ -247,8 +247,8 @@ int doci_op_time_sigma(doci_op *op, int prec, int reps, double *ms)
     hipEvent_t e0, e1;
     int rc = DOCI_OK;
     if (reps < 1) return doci_fail(DOCI_E_ARG, "reps");
-    HIPCHK(hipMalloc(&x, es * op->n));
-    HIPCHK(hipMalloc(&y, es * op->n));
+    HIPCHK(dh_dmalloc(&x, es * op->n));
+    HIPCHK(dh_dmalloc(&y, es * op->n));
     HIPCHK(hipEventCreate(&e0));
     HIPCHK(hipEventCreate(&e1));
     HIPCHK(hipMemsetAsync(x, 0, es * op->n, op->stream));
@@ -264,8 +264,9 @@ int doci_op_time_sigma(doci_op *op, int prec, int reps, double *ms)
     }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    hipFree(x);
-    hipFree(y);
+    hipStreamSynchronize(op->stream);
+    dh_dfree(x);
+    dh_dfree(y);
     return rc;
 }
 

@@ -65,7 +65,7 @@ int doci_solve(doci_op *op, const doci_solve_opts *o, const double *guess, doubl
     dh_ctx c;
     dh_ctx_of_op(op, &c);
     double *d_x = NULL;
-    HIPCHK(hipMalloc((void **)&d_x, sizeof(double) * op->n));
+    HIPCHK(dh_dmalloc((void **)&d_x, sizeof(double) * op->n));
     int rc;
     if (guess) {
         hipError_t e = hipMemcpyAsync(d_x, guess, sizeof(double) * op->n, hipMemcpyHostToDevice,
@@ -81,7 +81,8 @@ int doci_solve(doci_op *op, const doci_solve_opts *o, const double *guess, doubl
         if (e == hipSuccess) e = hipStreamSynchronize(c.stream);
         if (e != hipSuccess) rc = doci_fail(DOCI_E_HIP, "result download: %s", hipGetErrorString(e));
     }
-    hipFree(d_x);
+    hipStreamSynchronize(c.stream);
+    dh_dfree(d_x);
     return rc;
 }
 
@@ -126,7 +127,7 @@ static int solve_dropin(double *vec, int dims, double *energy,
         doci_op *op = dat;
         dh_ctx_of_op(op, &c);
         if (diagonal && diagonal != op->D_host) { /* caller-supplied diagonal */
-            HIPCHK(hipMalloc((void **)&d_Dtmp, sizeof(double) * dims));
+            HIPCHK(dh_dmalloc((void **)&d_Dtmp, sizeof(double) * dims));
             HIPCHK(hipMemcpy(d_Dtmp, diagonal, sizeof(double) * dims, hipMemcpyHostToDevice));
             c.d_D = d_Dtmp;
         }
@@ -137,29 +138,30 @@ static int solve_dropin(double *vec, int dims, double *energy,
         c.apply = apply_cb;
         c.user = &u;
         HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
-        HIPCHK(hipMalloc((void **)&c.d_red, sizeof(double) * dk_red_scratch_doubles(DK_MAX_VEC + 8)));
-        HIPCHK(hipMalloc((void **)&c.d_out, sizeof(double) * (DK_MAX_VEC + 8)));
-        HIPCHK(hipMalloc((void **)&c.d_idx, sizeof(int64_t)));
+        HIPCHK(dh_dmalloc((void **)&c.d_red, sizeof(double) * dk_red_scratch_doubles(DK_MAX_VEC + 8)));
+        HIPCHK(dh_dmalloc((void **)&c.d_out, sizeof(double) * (DK_MAX_VEC + 8)));
+        HIPCHK(dh_dmalloc((void **)&c.d_idx, sizeof(int64_t)));
         HIPCHK(hipHostMalloc((void **)&c.h_out, sizeof(double) * (DK_MAX_VEC + 8), 0));
         HIPCHK(hipHostMalloc((void **)&u.h_in, sizeof(double) * dims, 0));
         HIPCHK(hipHostMalloc((void **)&u.h_out, sizeof(double) * dims, 0));
-        HIPCHK(hipMalloc((void **)&d_Dtmp, sizeof(double) * dims));
+        HIPCHK(dh_dmalloc((void **)&d_Dtmp, sizeof(double) * dims));
         if (diagonal)
             HIPCHK(hipMemcpy(d_Dtmp, diagonal, sizeof(double) * dims, hipMemcpyHostToDevice));
         else
             HIPCHK(hipMemset(d_Dtmp, 0, sizeof(double) * dims));
         c.d_D = d_Dtmp;
     }
-    HIPCHK(hipMalloc((void **)&d_x, sizeof(double) * dims));
+    HIPCHK(dh_dmalloc((void **)&d_x, sizeof(double) * dims));
     HIPCHK(hipMemcpy(d_x, vec, sizeof(double) * dims, hipMemcpyHostToDevice));
     rc = run_solver(&c, o, d_x, energy, info);
     if (rc >= 0) HIPCHK(hipMemcpy(vec, d_x, sizeof(double) * dims, hipMemcpyDeviceToHost));
-    hipFree(d_x);
-    if (d_Dtmp) hipFree(d_Dtmp);
+    hipStreamSynchronize(c.stream);
+    dh_dfree(d_x);
+    dh_dfree(d_Dtmp);
     if (own_ctx) {
-        hipFree(c.d_red);
-        hipFree(c.d_out);
-        hipFree(c.d_idx);
+        dh_dfree(c.d_red);
+        dh_dfree(c.d_out);
+        dh_dfree(c.d_idx);
         hipHostFree(c.h_out);
         hipHostFree(u.h_in);
         hipHostFree(u.h_out);

@@ -74,8 +74,8 @@ void dh_stars_free(doci_op *op)
 {
     dh_stars *s = op->stars;
     if (!s) return;
-    if (s->d_bad) hipFree(s->d_bad);
-    if (s->Z) hipFree(s->Z);
+    dh_dfree(s->d_bad);
+    dh_dfree(s->Z);
     free(s);
     op->stars = NULL;
 }
@@ -438,10 +438,10 @@ int dh_stars_build(doci_op *op, unsigned lds_budget, unsigned sgmax, unsigned ma
     if (!s) return doci_fail(DOCI_E_NOMEM, "star descriptor");
     s->sh = sh;
     s->pass = (int)sh->d.pass;
-    hipError_t e = hipMalloc((void **)&s->d_bad, sizeof(uint32_t));
+    hipError_t e = dh_dmalloc((void **)&s->d_bad, sizeof(uint32_t));
     if (e == hipSuccess) e = hipMemset(s->d_bad, 0, sizeof(uint32_t));
     if (e != hipSuccess) {
-        if (s->d_bad) hipFree(s->d_bad);
+        dh_dfree(s->d_bad);
         free(s);
         return doci_fail(DOCI_E_HIP, "star counter: %s", hipGetErrorString(e));
     }
@@ -453,7 +453,7 @@ int dh_stars_sigma(doci_op *op, const void *in, void *out, int prec, double shif
 {
     dh_stars *s = op->stars;
     if (!s->Z) { /* the slot buffer is allocated on first use and kept */
-        hipError_t e = hipMalloc(&s->Z, sizeof(double) * (s->sh->zelems ? s->sh->zelems : 1));
+        hipError_t e = dh_dmalloc(&s->Z, sizeof(double) * (s->sh->zelems ? s->sh->zelems : 1));
         if (e != hipSuccess) {
             s->Z = NULL;
             return doci_fail(DOCI_E_NOMEM, "sigma slot buffer (%zu entries): %s", s->sh->zelems,
