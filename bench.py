@@ -173,7 +173,8 @@ def main() -> int:
                     help="requests of the serving leg (0 skips it)")
     ap.add_argument("--serve-rates", default="1,5,10",
                     help="comma-separated arrival rates as multiples of mu0")
-    ap.add_argument("--serve-precision", default="fp64", choices=["fp64", "mixed"])
+    ap.add_argument("--serve-precision", default="mixed", choices=["fp64", "mixed"],
+                    help="precision flag of every served request (the baseline is plain FP64)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: the DOCI solve here is a single-GPU "

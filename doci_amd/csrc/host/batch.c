@@ -508,7 +508,9 @@ static int admit(doci_batch *b, int ri, int slot)
     bi->D = D;
     bi->slots = d + tb + vs;
     bi->slot_stride = vs;
-    bi->x = r->dx;
+    /* the Ritz vector is written only where it is read: kept vectors, or the
+     * FP64 restart of a mixed-precision instance */
+    bi->x = (b->o.keep_vectors || r->want_mixed) ? r->dx : NULL;
     r->prec = r->want_mixed ? 1 : 0;
     /* the descriptor is staged too: r->bi may move before the copy runs */
     char *hb = NULL;

@@ -226,7 +226,7 @@ __device__ __forceinline__ void resid_rows(const T *const (&V)[M], const T *cons
         const double tval = d > 1e-12 ? r / d : r / 1e-12;
         set_el<T, W>(tv, e, (T)tval);
         td[e] = el<T, W>(tv, e);
-        x[i] = xs[e];
+        if (x) x[i] = xs[e];
         acc[0] += r * r;
         acc[1] += td[e] * td[e];
     }
