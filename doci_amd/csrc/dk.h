@@ -85,7 +85,7 @@ typedef struct {
     uint32_t nsgroup;           /* star-pass groups */
     uint32_t sgmax;             /* stars per star group, at most */
     uint32_t selems;            /* LDS elements staged per star group, at most */
-    uint32_t ymax;              /* largest number of members of a star */
+    uint32_t ymax;              /* member-table entries per star (= a) */
     uint32_t pass;              /* star pass: 0 column pass (wave-uniform K), 1 per-star vector,
                                    2 per-star FP64 matrix cores, 3 FP32 column pass on the matrix cores */
     const uint32_t *slab_start; /* per slab (order: k, then lexicographic alpha) */

@@ -66,7 +66,8 @@ static size_t star_lds_bytes(unsigned L, unsigned N, unsigned sgmax, unsigned ym
 {
 #define A8(x) (((size_t)(x) + 7) & ~(size_t)7)
     return A8(8 * (size_t)L * L) + A8(4 * (size_t)(L + 1) * (N + 2)) + 2 * A8(8 * (size_t)selems) +
-           2 * A8(4 * (size_t)sgmax * ymax) + 2 * A8(4 * (size_t)sgmax) + A8((size_t)sgmax * ymax);
+           2 * A8(4 * (size_t)sgmax * ymax) + 2 * A8(4 * (size_t)sgmax) +
+           A8((size_t)sgmax * (ymax > 8 ? ymax : 8));
 #undef A8
 }
 
@@ -199,12 +200,11 @@ static int build_shape(const dk_tables *dt, unsigned lds_budget, unsigned sgmax,
     for (; b >= 1; --b) {
         a = L - b;
         uint64_t mx = 0; /* biggest centre slab */
-        ymax = 1;
+        ymax = a; /* member tables: a entries per star (the column pass indexes them by orbital) */
         for (unsigned j = 0; j < N; ++j) {
             if (!valid_sector(j + 1, N, a, b)) continue; /* members' sector */
             const uint64_t st = valid_sector(j, N, a, b) ? dh_binom(N - j, b) : 0;
             if (st > mx) mx = st;
-            if (a - j > ymax) ymax = a - j;
         }
         uint64_t maxS = 0; /* biggest slab: the main pass stages whole slabs */
         for (unsigned k = 0; k <= N; ++k)

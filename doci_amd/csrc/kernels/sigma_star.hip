@@ -88,7 +88,7 @@ __host__ __device__ size_t star_lds(uint32_t L, uint32_t N, const dk_stars &st, 
     char *mz = take(sizeof(uint32_t) * st.sgmax * st.ymax);
     char *cs = take(sizeof(uint32_t) * st.sgmax);
     char *cz = take(sizeof(uint32_t) * st.sgmax);
-    char *yc = take(st.sgmax * st.ymax);
+    char *yc = take(st.sgmax * (st.ymax > 8 ? st.ymax : 8)); // also the stars' A-parts (column pass)
     if (p) {
         p->K = (double *)K;
         p->bt = (uint32_t *)bt;
@@ -134,7 +134,10 @@ struct StarCtx {
 
 // Stages K, the binomials, the member tables and the centre slabs of star
 // group sg in LDS (zeroing the centre accumulators); ends with a barrier.
-template <typename T, bool kChecked, int NT = kBlock, typename OK>
+// kByY (the column pass): no K, and the member tables indexed by orbital,
+// entry s * a + y for every y < a (entries of y in rho are not written),
+// instead of by member number s * ny + yi.
+template <typename T, bool kChecked, int NT = kBlock, bool kByY = false, typename OK>
 __device__ __forceinline__ StarCtx star_setup(const dk_tables &t, const dk_stars &st, uint32_t sg,
                                               const T *__restrict__ c, StarLds &p, OK &&ok)
 {
@@ -147,7 +150,8 @@ __device__ __forceinline__ StarCtx star_setup(const dk_tables &t, const dk_stars
     x.ztot = st.zoff[x.a + 1];
     const int L = x.L, N = x.N, a = x.a, b = x.b;
     const uint32_t n = x.n;
-    for (int i = threadIdx.x; i < L * L; i += NT) p.K[i] = t.K[i];
+    if (!kByY)
+        for (int i = threadIdx.x; i < L * L; i += NT) p.K[i] = t.K[i];
     for (int i = threadIdx.x; i < (L + 1) * (N + 2); i += NT) p.bt[i] = t.binom[i];
     const uint32_t first = st.sgroup[2 * sg], w = st.sgroup[2 * sg + 1];
     const int nst = (int)(w & 0xffffu), j = (int)(w >> 16);
@@ -172,7 +176,60 @@ __device__ __forceinline__ StarCtx star_setup(const dk_tables &t, const dk_stars
     const int nsm = nslot_of(km, mm, a), nsc = nslot_of(j, mc, a);
     const uint64_t amask_all = a < 64 ? ((1ull << a) - 1) : ~0ull;
 
+    // centre slab starts first: the centre loads below are then in flight
+    // while the member tables are computed
+    // (kByY: the stars' A-parts go to LDS too, so that no global load is
+    // waited for between the centre loads and their use)
+    uint64_t *srho = (uint64_t *)p.ycol;
+    for (int s = threadIdx.x; s < nst; s += NT) {
+        const uint64_t rho = st.star_amask[first + s];
+        if (kByY) srho[s] = rho;
+        if (has_ctr) {
+            p.ctr_start[s] = slab_first(rho, mc, L, N, b, n, C);
+            p.ctr_zb[s] = zc_sector + lex_a(rho, j, a, C) * (uint32_t)(nsc * Sc) + (uint32_t)(j * Sc);
+        }
+    }
+    __syncthreads();
+    constexpr int CPT = 8; // centre loads in flight per lane
+    const int nce = has_ctr ? nst * Sc : 0;
+    T cv[CPT];
+    auto ctr_load = [&](int e0) {
+#pragma unroll
+        for (int u = 0; u < CPT; ++u) {
+            const int e = e0 + u * NT + (int)threadIdx.x;
+            cv[u] = (T)0;
+            if (e < nce) {
+                const int s = e / Sc;
+                const uint32_t gi = p.ctr_start[s] + (uint32_t)(e - s * Sc);
+                if (ok(gi, n)) cv[u] = c[gi];
+            }
+        }
+    };
+    T *ctrv = (T *)p.ctr;
+    auto ctr_put = [&](int e0) {
+#pragma unroll
+        for (int u = 0; u < CPT; ++u) {
+            const int e = e0 + u * NT + (int)threadIdx.x;
+            if (e < nce) {
+                ctrv[e] = cv[u];
+                p.acc[e] = 0.0;
+            }
+        }
+    };
+    ctr_load(0);
+
     // member tables: first rank, Z position of this star's slot, orbital y
+    if (kByY) {
+        for (int task = threadIdx.x; task < nst * a; task += NT) {
+            const int s = task / a, y = task - s * a;
+            const uint64_t rho = srho[s];
+            if ((rho >> y) & 1) continue;
+            const uint64_t mem = rho | (1ull << y);
+            const int pos = __popcll(rho & ((1ull << y) - 1));
+            p.mem_start[task] = slab_first(mem, mm, L, N, b, n, C);
+            p.mem_zb[task] = zm_sector + lex_a(mem, km, a, C) * (uint32_t)(nsm * Sm) + (uint32_t)(pos * Sm);
+        }
+    } else
     for (int task = threadIdx.x; task < nst * ny; task += NT) {
         const int s = task / ny, yi = task - s * ny;
         const uint64_t rho = st.star_amask[first + s];
@@ -186,34 +243,11 @@ __device__ __forceinline__ StarCtx star_setup(const dk_tables &t, const dk_stars
             zm_sector + lex_a(mem, km, a, C) * (uint32_t)(nsm * Sm) + (uint32_t)(pos * Sm);
         p.ycol[s * ny + yi] = (uint8_t)y;
     }
-    if (has_ctr)
-        for (int s = threadIdx.x; s < nst; s += NT) {
-            const uint64_t rho = st.star_amask[first + s];
-            p.ctr_start[s] = slab_first(rho, mc, L, N, b, n, C);
-            p.ctr_zb[s] = zc_sector + lex_a(rho, j, a, C) * (uint32_t)(nsc * Sc) + (uint32_t)(j * Sc);
-        }
-    __syncthreads();
-    T *ctrv = (T *)p.ctr;
-    if (has_ctr)
-        for (int e0 = threadIdx.x; e0 < nst * Sc; e0 += 4 * NT) {
-            T v[4]; // four loads in flight per lane
-#pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int e = e0 + u * NT;
-                v[u] = (T)0;
-                if (e < nst * Sc) {
-                    const int s = e / Sc;
-                    const uint32_t gi = p.ctr_start[s] + (uint32_t)(e - s * Sc);
-                    if (ok(gi, n)) v[u] = c[gi];
-                }
-            }
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-                if (e0 + u * NT < nst * Sc) {
-                    ctrv[e0 + u * NT] = v[u];
-                    p.acc[e0 + u * NT] = 0.0;
-                }
-        }
+    ctr_put(0);
+    for (int e0 = CPT * NT; e0 < nce; e0 += CPT * NT) {
+        ctr_load(e0);
+        ctr_put(e0);
+    }
     __syncthreads();
     return x;
 }
@@ -577,10 +611,9 @@ __global__ __launch_bounds__(kBlock, 3) void k_star_col(dk_tables t, dk_stars st
         atomicAdd(bad, 1u);
         return false;
     };
-    const StarCtx x = star_setup<T, kChecked>(t, st, sg, c, p, ok);
+    const StarCtx x = star_setup<T, kChecked, kBlock, true>(t, st, sg, c, p, ok);
     // star-group constants (wave-uniform; some were read from LDS)
     const int a = (int)st.a, b = (int)st.b;
-    const int ny = __builtin_amdgcn_readfirstlane(x.ny);
     const int Sm = __builtin_amdgcn_readfirstlane(x.Sm);
     const int Sc = __builtin_amdgcn_readfirstlane(x.Sc);
     const int nst = __builtin_amdgcn_readfirstlane(x.nst);
@@ -589,7 +622,6 @@ __global__ __launch_bounds__(kBlock, 3) void k_star_col(dk_tables t, dk_stars st
         if (kChecked && threadIdx.x == 0) atomicAdd(bad, 1u);
         return;
     }
-    const uint32_t first = st.sgroup[2 * sg];
     const int ncol = nst * Sm;
     const uint64_t amask = a < 64 ? ((1ull << a) - 1) : ~0ull;
     const int lane = (int)(threadIdx.x & 63);
@@ -602,7 +634,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_star_col(dk_tables t, dk_stars st
         const bool live = col < ncol;
         const int s = live ? col / Sm : 0;
         const int beta = live ? col - s * Sm : 0;
-        const uint64_t rho = live ? st.star_amask[first + s] : amask;
+        const uint64_t rho = live ? ((const uint64_t *)p.ycol)[s] : amask; // staged by star_setup
         // orbitals in rho for every lane's star: skipped by the whole wave
         uint32_t clo = (uint32_t)rho, chi = (uint32_t)(rho >> 32);
 #pragma unroll
@@ -612,8 +644,9 @@ __global__ __launch_bounds__(kBlock, 3) void k_star_col(dk_tables t, dk_stars st
         }
         const uint64_t common = ((uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)chi) << 32) |
                                 (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)clo);
-        const uint32_t *ms = p.mem_start + s * ny;
-        const uint32_t *mz = p.mem_zb + s * ny;
+        // member tables of this lane's star, indexed by orbital
+        const uint32_t *ms = p.mem_start + s * a;
+        const uint32_t *mz = p.mem_zb + s * a;
         // member values X_y, all loads in flight at once, coalesced across the
         // lanes of one star.  Lanes without member y (y in rho, y >= a, or no
         // column) read a zero instead: KP[0] is a zero diagonal entry, which
@@ -624,8 +657,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_star_col(dk_tables t, dk_stars st
 #pragma unroll
         for (int y = 0; y < AMAX; ++y) {
             const bool mem = !((rhox >> y) & 1);
-            const int yi = mem ? y - __popcll(rho & ((1ull << y) - 1)) : 0;
-            const uint32_t gi = ms[yi] + (uint32_t)beta;
+            const uint32_t gi = ms[y] + (uint32_t)beta; // read for every y: the table is in LDS
             v[y] = load_global(mem && ok(gi, x.n) ? c + gi : zero);
         }
         // out[r]: row r < AMAX is member row x = r, row AMAX + q the centre row
@@ -688,8 +720,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_star_col(dk_tables t, dk_stars st
         for (int xo = 0; xo < AMAX; ++xo) {
             if (xo < a && !((common >> xo) & 1)) {
                 if (live && !((rho >> xo) & 1)) {
-                    const int xi = xo - __popcll(rho & ((1ull << xo) - 1));
-                    const uint32_t zi = mz[xi] + (uint32_t)beta;
+                    const uint32_t zi = mz[xo] + (uint32_t)beta;
                     if (ok(zi, x.ztot)) Z[zi] = (T)out[xo];
                 }
             }
