@@ -660,6 +660,19 @@ __global__ __launch_bounds__(kBlock, 3) void k_star_col(dk_tables t, dk_stars st
             const uint32_t gi = ms[y] + (uint32_t)beta; // read for every y: the table is in LDS
             v[y] = load_global(mem && ok(gi, x.n) ? c + gi : zero);
         }
+        // the centre's insert offsets lexB(beta + t), loaded while the member
+        // values arrive and the A products run
+        uint32_t offs[BMAX];
+        if (has_ctr) {
+            // per-block copies of b and Sm: comparisons and offsets derived
+            // from them are recomputed here instead of held across the loop
+            const int bq = opaque(b, c0), Smq = opaque(Sm, c0);
+#pragma unroll
+            for (int q = 0; q < BMAX; ++q) {
+                const uint16_t *row = x.ins + (size_t)(q * Smq); // uniform row base
+                offs[q] = q < bq ? (uint32_t)row[(uint32_t)beta] : 0xffffu;
+            }
+        }
         // out[r]: row r < AMAX is member row x = r, row AMAX + q the centre row
         // of B orbital a + q
         double out[R];
@@ -682,20 +695,12 @@ __global__ __launch_bounds__(kBlock, 3) void k_star_col(dk_tables t, dk_stars st
             }
         }
         // hops A -> B through the centre values, and the centre's hops B -> A:
-        // the insert offsets lexB(beta + t) and centre values first (all loads
-        // in flight together), the accumulator updates, then the products
+        // the centre values first (all loads in flight together), the
+        // accumulator updates, then the products
         if (has_ctr) {
-            // per-block copies of b and Sm: comparisons and offsets derived
-            // from them are recomputed here instead of held across the loop
-            const int bq = opaque(b, c0), Smq = opaque(Sm, c0);
+            const int bq = opaque(b, c0);
             const T *ctrv = (const T *)p.ctr + s * Sc;
             double *cacc = p.acc + s * Sc;
-            uint32_t offs[BMAX];
-#pragma unroll
-            for (int q = 0; q < BMAX; ++q) {
-                const uint16_t *row = x.ins + (size_t)(q * Smq); // uniform row base
-                offs[q] = q < bq ? (uint32_t)row[(uint32_t)beta] : 0xffffu;
-            }
             uint32_t on = 0;
             double cv[BMAX];
 #pragma unroll
