@@ -97,7 +97,7 @@ typedef struct {
     uint32_t jgroup[DK_MAX_L + 1]; /* star groups of j: [jgroup[j], jgroup[j+1]), j < N */
     /* Tables over the masks beta of m pairs in B (b <= 16), r = lexB(beta):
      * bmask[bmask_off[m] + r] = beta;
-     * hop[hop_off[m] + h * C(b,m) + r] = J | (o * b + t) << 16 for the h-th
+     * hop[hop_off[m] + h * C(b,m) + r] = J | 8 (o * b + t) << 16 for the h-th
      *   hop beta - o + t (o in beta ascending, then t not in beta ascending),
      *   J = its lexB rank; rows h from m(b-m) up to the next multiple of 8
      *   are padding, r | (b * b) << 16 (index b * b names a zero K entry);

@@ -149,12 +149,12 @@ static int build_b_tables(star_shape *s, unsigned b)
                     }
                     jdx[pos] = (int)t;
                     const uint32_t J = lex_rank(jdx, m, b);
-                    hop[s->d.hop_off[m] + hh * S + r] = J | (uint32_t)(o * b + t) << 16;
+                    hop[s->d.hop_off[m] + hh * S + r] = J | (uint32_t)(8 * (o * b + t)) << 16;
                     ++hh;
                 }
             }
             for (; hh < hop_rows(m, b); ++hh) /* padding: a hop onto itself with K = 0 */
-                hop[s->d.hop_off[m] + hh * S + r] = (uint32_t)r | (uint32_t)(b * b) << 16;
+                hop[s->d.hop_off[m] + hh * S + r] = (uint32_t)r | (uint32_t)(8 * b * b) << 16;
             if (m < b)
                 for (unsigned t = 0; t < b; ++t) {
                     uint16_t v = 0xffff;

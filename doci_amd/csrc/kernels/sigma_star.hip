@@ -984,13 +984,21 @@ __global__ __launch_bounds__(kBlock) void k_star_main(dk_tables t, dk_stars st,
                 hv[i] = __builtin_amdgcn_raw_buffer_load_b32(hrsrc, (int)voff, (h0 + i) * S * 4, 0);
 #pragma unroll
             for (int i = 0; i < kHB; ++i) {
-                const uint32_t J = hv[i] & 0xffffu, kx = hv[i] >> 16;
-                if (ok(J, (uint32_t)S) && ok(kx, (uint32_t)(b * b + 1))) acc += Kbb[kx] * (double)os[J];
+                // J, and the byte offset of K in Kbb (which starts the LDS)
+                const uint32_t J = hv[i] & 0xffffu, kb = hv[i] >> 16;
+                if (ok(J, (uint32_t)S) && ok(kb, (uint32_t)(8 * (b * b + 1))))
+                    acc += *(const double *)((const char *)Kbb + kb) * (double)os[J];
             }
         }
+        // (slots past ns hold zeros: the second half is skipped when the
+        // group has at most kZS / 2 slots, a uniform branch)
         double zs = 0;
 #pragma unroll
-        for (int q = 0; q < kZS; ++q) zs += (double)zv[q];
+        for (int q = 0; q < kZS / 2; ++q) zs += (double)zv[q];
+        if (ns > kZS / 2) {
+#pragma unroll
+            for (int q = kZS / 2; q < kZS; ++q) zs += (double)zv[q];
+        }
         for (int q = kZS; q < ns; ++q) {
             const uint32_t zi = zb + (uint32_t)(q * S);
             if (ok(zi, ztot)) zs += (double)Z[zi];
