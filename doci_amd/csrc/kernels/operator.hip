@@ -27,9 +27,9 @@ __global__ __launch_bounds__(kBlock) void k_diagonal(dk_tables t, double *__rest
 {
     extern __shared__ double lds[];
     const int L = (int)t.L, N = (int)t.N, N2 = N + 2, LL = L * L;
-    double *J = lds, *X = J + LL, *h = X + LL;
+    double *J2 = lds, *X = J2 + LL, *h = X + LL; // J2 = 2 J (exact)
     uint32_t *bt = (uint32_t *)(h + L);
-    stage_f64(J, t.J, LL);
+    for (int i = threadIdx.x; i < LL; i += blockDim.x) J2[i] = 2 * t.J[i];
     stage_f64(X, t.X, LL);
     stage_f64(h, t.h, L);
     stage_u32(bt, t.binom, (L + 1) * N2);
@@ -44,12 +44,11 @@ __global__ __launch_bounds__(kBlock) void k_diagonal(dk_tables t, double *__rest
         const int o = 63 - __clzll((long long)rem);
         rem ^= 1ull << o;
         acc += 2 * h[o];
-        const double *Jo = J + o * L, *Xo = X + o * L;
-        for (int q = 0; q < L; ++q) {
-            const int is = (int)((m >> q) & 1);
-            acc += (2 * is) * Jo[q];
-            acc += (1 - is) * Xo[q];
-        }
+        // per q the reference adds 2 is J then (1 - is) X (doci.c:615,619);
+        // one of the two is an exact zero that leaves acc unchanged, so a
+        // single add of the other gives the same bits
+        const double *Jo = J2 + o * L, *Xo = X + o * L;
+        for (int q = 0; q < L; ++q) acc += ((m >> q) & 1) ? Jo[q] : Xo[q];
     }
     D[I] = acc;
 }
