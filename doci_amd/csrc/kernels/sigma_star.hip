@@ -653,12 +653,19 @@ __global__ __launch_bounds__(kBlock, 3) void k_star_col(dk_tables t, dk_stars st
         // reads as 0 in either precision.
         const T *zero = (const T *)kp;
         const uint64_t rhox = live ? (rho | ~amask) : ~0ull;
+        // Orbitals outside the wave's set ym (in every lane's rho, or >= a)
+        // are skipped by the whole wave; for FP64 their loads too (for FP32
+        // the branches cost more registers than the loads they save).
+        const uint64_t ym = ~common & amask; // the wave's orbitals y
         T v[AMAX];
 #pragma unroll
         for (int y = 0; y < AMAX; ++y) {
-            const bool mem = !((rhox >> y) & 1);
-            const uint32_t gi = ms[y] + (uint32_t)beta; // read for every y: the table is in LDS
-            v[y] = load_global(mem && ok(gi, x.n) ? c + gi : zero);
+            v[y] = (T)0;
+            if (sizeof(T) == 4 || ((ym >> y) & 1)) {
+                const bool mem = !((rhox >> y) & 1);
+                const uint32_t gi = ms[y] + (uint32_t)beta; // read for every lane: the table is in LDS
+                v[y] = load_global(mem && ok(gi, x.n) ? c + gi : zero);
+            }
         }
         // the centre's insert offsets lexB(beta + t), loaded while the member
         // values arrive and the A products run
@@ -679,7 +686,6 @@ __global__ __launch_bounds__(kBlock, 3) void k_star_col(dk_tables t, dk_stars st
 #pragma unroll
         for (int r = 0; r < R; ++r) out[r] = 0.0;
         const double *kctr = kp + AMAX * R; // centre columns
-        const uint64_t ym = ~common & amask; // the wave's orbitals y
         double kbuf[kKC];
         {
             const double *k0 = ym ? kp + (__ffsll((unsigned long long)ym) - 1) * R : kctr;
