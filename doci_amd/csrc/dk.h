@@ -94,6 +94,8 @@ typedef struct {
     const uint64_t *star_amask; /* stars (order: j, then lexicographic rho) */
     const uint32_t *sgroup;     /* star groups: first star, nstar | j << 16 */
     const uint32_t *zoff;       /* [a + 2]: first Z index of each k-sector */
+    const uint32_t *mtab;       /* per star and orbital y < a, pairs: first rank and Z position
+                                   of the slot of member slab rho + y (0, 0 for y in rho) */
     uint32_t jgroup[DK_MAX_L + 1]; /* star groups of j: [jgroup[j], jgroup[j+1]), j < N */
     /* Tables over the masks beta of m pairs in B (b <= 16), r = lexB(beta):
      * bmask[bmask_off[m] + r] = beta;

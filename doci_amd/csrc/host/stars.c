@@ -246,10 +246,17 @@ static int build_shape(const dk_tables *dt, unsigned lds_budget, unsigned sgmax,
     uint32_t *sstart = malloc(sizeof(uint32_t) * nslab), *szb = malloc(sizeof(uint32_t) * nslab);
     sgrp_t *grp = malloc(sizeof(sgrp_t) * (nslab + 1));
     uint64_t *samask = malloc(sizeof(uint64_t) * (nstar + 1));
+    /* member table: for star rho and orbital y < a not in rho, the first
+     * rank and the Z position of its slot of member slab rho + y */
+    uint32_t *mtab = malloc(sizeof(uint32_t) * 2 * (nstar * a + 1));
+    uint64_t secbase[DK_MAX_L + 2] = {0}; /* first slab of each k-sector */
+    uint64_t cb[65][65];                  /* cb[n][k] = C(n, k), n <= 64 */
+    for (unsigned x = 0; x <= 64; ++x)
+        for (unsigned q = 0; q <= 64; ++q) cb[x][q] = q > x ? 0 : (q == 0 || q == x) ? 1 : cb[x - 1][q - 1] + cb[x - 1][q];
     uint32_t *sgw = malloc(sizeof(uint32_t) * 2 * (nstar + 1)), *gw = NULL;
     star_shape *s = NULL;
     int idx[64];
-    if (!sstart || !szb || !grp || !samask || !sgw) {
+    if (!sstart || !szb || !grp || !samask || !sgw || !mtab) {
         rc = doci_fail(DOCI_E_NOMEM, "star tables");
         goto out;
     }
@@ -259,6 +266,7 @@ static int build_shape(const dk_tables *dt, unsigned lds_budget, unsigned sgmax,
         if (!valid_sector(k, N, a, b)) continue;
         const unsigned m = N - k;
         const uint64_t S = dh_binom(m, b), ns = nslot(k, m, a);
+        secbase[k] = is;
         for (unsigned e = 0; e < k; ++e) idx[e] = (int)e;
         uint32_t gfirst = (uint32_t)is, gcount = 0;
         uint64_t li = 0;
@@ -313,6 +321,29 @@ static int build_shape(const dk_tables *dt, unsigned lds_budget, unsigned sgmax,
                 gfirst = (uint32_t)ist;
                 gcount = 0;
             }
+            /* members rho + y: slab index secbase[j + 1] + lexicographic rank
+             * of rho + y, C(a,k) - 1 - sum_e C(a-1-x_e, k-e) over its sorted
+             * orbitals x_e (k = j + 1): rho's orbitals below y keep their
+             * place e, those above move to e + 1 (prefix and suffix sums) */
+            const unsigned k = j + 1;
+            const uint64_t Sm = cb[b][N - k];
+            uint64_t pre[65], suf[65];
+            pre[0] = 0;
+            for (unsigned e = 0; e < j; ++e) pre[e + 1] = pre[e] + cb[a - 1 - idx[e]][k - e];
+            suf[j] = 0;
+            for (unsigned e = j; e-- > 0;) suf[e] = suf[e + 1] + cb[a - 1 - idx[e]][k - 1 - e];
+            for (unsigned y = 0, pos = 0; y < a; ++y) {
+                uint32_t *mt = mtab + 2 * (ist * a + y);
+                if ((msk >> y) & 1) {
+                    mt[0] = mt[1] = 0; /* not a member */
+                    ++pos;
+                    continue;
+                }
+                const uint64_t li = cb[a][k] - 1 - (pre[pos] + cb[a - 1 - y][k - pos] + suf[pos]);
+                const uint64_t si = secbase[k] + li;
+                mt[0] = sstart[si];
+                mt[1] = (uint32_t)(szb[si] + pos * Sm);
+            }
             samask[ist++] = msk;
             ++gcount;
         } while (next_comb(idx, j, a));
@@ -329,7 +360,9 @@ static int build_shape(const dk_tables *dt, unsigned lds_budget, unsigned sgmax,
         const size_t b_am = sizeof(uint64_t) * nstar, b_ss = sizeof(uint32_t) * nslab,
                      b_zb = sizeof(uint32_t) * nslab, b_g = sizeof(uint32_t) * 2 * ng,
                      b_sg = sizeof(uint32_t) * 2 * nsg, b_zo = sizeof(uint32_t) * (a + 2);
-        hipError_t e = hipMalloc(&s->dmem, b_am + b_ss + b_zb + b_g + b_sg + b_zo);
+        const size_t o_mt = (b_am + b_ss + b_zb + b_g + b_sg + b_zo + 7) & ~(size_t)7,
+                     b_mt = sizeof(uint32_t) * 2 * nstar * a;
+        hipError_t e = hipMalloc(&s->dmem, o_mt + b_mt);
         char *d = s->dmem;
         if (e == hipSuccess) e = hipMemcpy(d, samask, b_am, hipMemcpyHostToDevice);
         if (e == hipSuccess) e = hipMemcpy(d + b_am, sstart, b_ss, hipMemcpyHostToDevice);
@@ -339,6 +372,7 @@ static int build_shape(const dk_tables *dt, unsigned lds_budget, unsigned sgmax,
             e = hipMemcpy(d + b_am + b_ss + b_zb + b_g, sgw, b_sg, hipMemcpyHostToDevice);
         if (e == hipSuccess)
             e = hipMemcpy(d + b_am + b_ss + b_zb + b_g + b_sg, zoff, b_zo, hipMemcpyHostToDevice);
+        if (e == hipSuccess) e = hipMemcpy(d + o_mt, mtab, b_mt, hipMemcpyHostToDevice);
         if (e != hipSuccess) {
             rc = doci_fail(DOCI_E_HIP, "star tables on device: %s", hipGetErrorString(e));
             goto out;
@@ -360,6 +394,7 @@ static int build_shape(const dk_tables *dt, unsigned lds_budget, unsigned sgmax,
         s->d.group = (const uint32_t *)(d + b_am + b_ss + b_zb);
         s->d.sgroup = (const uint32_t *)(d + b_am + b_ss + b_zb + b_g);
         s->d.zoff = (const uint32_t *)(d + b_am + b_ss + b_zb + b_g + b_sg);
+        s->d.mtab = (const uint32_t *)(d + o_mt);
         s->zelems = zacc + dk_star_kp_doubles(a, b); /* slots, then packed K */
         s->lim[0] = lds_budget;
         s->lim[1] = sgmax;
@@ -387,6 +422,7 @@ out:
     free(szb);
     free(grp);
     free(samask);
+    free(mtab);
     free(sgw);
     free(gw);
     return rc;

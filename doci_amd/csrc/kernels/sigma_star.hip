@@ -135,8 +135,8 @@ struct StarCtx {
 // Stages K, the binomials, the member tables and the centre slabs of star
 // group sg in LDS (zeroing the centre accumulators); ends with a barrier.
 // kByY (the column pass): no K, and the member tables indexed by orbital,
-// entry s * a + y for every y < a (entries of y in rho are not written),
-// instead of by member number s * ny + yi.
+// entry s * a + y for every y < a (zero for y in rho), copied from the
+// host-built st.mtab instead of computed by member number s * ny + yi.
 template <typename T, bool kChecked, int NT = kBlock, bool kByY = false, typename OK>
 __device__ __forceinline__ StarCtx star_setup(const dk_tables &t, const dk_stars &st, uint32_t sg,
                                               const T *__restrict__ c, StarLds &p, OK &&ok)
@@ -177,9 +177,8 @@ __device__ __forceinline__ StarCtx star_setup(const dk_tables &t, const dk_stars
     const uint64_t amask_all = a < 64 ? ((1ull << a) - 1) : ~0ull;
 
     // centre slab starts first: the centre loads below are then in flight
-    // while the member tables are computed
-    // (kByY: the stars' A-parts go to LDS too, so that no global load is
-    // waited for between the centre loads and their use)
+    // while the member tables are built (kByY: the stars' A-parts go to LDS
+    // too, for the column loop)
     uint64_t *srho = (uint64_t *)p.ycol;
     for (int s = threadIdx.x; s < nst; s += NT) {
         const uint64_t rho = st.star_amask[first + s];
@@ -219,15 +218,12 @@ __device__ __forceinline__ StarCtx star_setup(const dk_tables &t, const dk_stars
     ctr_load(0);
 
     // member tables: first rank, Z position of this star's slot, orbital y
-    if (kByY) {
+    if (kByY) { // built on the host (st.mtab), contiguous for the group's stars
+        const uint2 *mt = (const uint2 *)st.mtab + (size_t)first * a;
         for (int task = threadIdx.x; task < nst * a; task += NT) {
-            const int s = task / a, y = task - s * a;
-            const uint64_t rho = srho[s];
-            if ((rho >> y) & 1) continue;
-            const uint64_t mem = rho | (1ull << y);
-            const int pos = __popcll(rho & ((1ull << y) - 1));
-            p.mem_start[task] = slab_first(mem, mm, L, N, b, n, C);
-            p.mem_zb[task] = zm_sector + lex_a(mem, km, a, C) * (uint32_t)(nsm * Sm) + (uint32_t)(pos * Sm);
+            const uint2 v = mt[task];
+            p.mem_start[task] = v.x;
+            p.mem_zb[task] = v.y;
         }
     } else
     for (int task = threadIdx.x; task < nst * ny; task += NT) {
