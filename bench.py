@@ -167,6 +167,7 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--calib", action="store_true",
                     help="first run a 1 GiB device copy (byte-counter calibration)")
+    ap.add_argument("--cgp", type=int, default=1, help="1: also solve the L=34 problem with CGP")
     ap.add_argument("--batch", type=int, default=512,
                     help="instances of the batched-solve leg (0 skips it)")
     ap.add_argument("--serve", type=int, default=2000,
@@ -265,6 +266,17 @@ def main() -> int:
                      "traffic_unit": "GB per sigma call (rocprof FETCH_SIZE + WRITE_SIZE, calibrated)",
                      "algorithmic_GB": round(bytes_per / 1e9, 3)},
     }
+    if args.cgp:
+        # the same integrals solved with CGP (src/CG.c), plain FP64, reported beside
+        # the Davidson figures (its stopping test is |g|^2 <= TOL, not |r| <= TOL)
+        t = time.perf_counter()
+        rc = op.solve("CGP", tol=1e-8, max_its=200, want_x=False)
+        tc = time.perf_counter() - t
+        out["cgp_fp64"] = {"seconds": round(tc, 4), "iterations": rc.info.iterations, "max_its": 200,
+                           "energy": rc.energy, "abs_dE_vs_davidson_fp64": abs(rc.energy - rp.energy),
+                           "sigma_calls": rc.info.sigma_calls}
+        log(f"cgp fp64: {tc:.3f} s its={rc.info.iterations} E={rc.energy:.12f} "
+            f"|dE| vs Davidson {abs(rc.energy - rp.energy):.1e}")
     if not args.no_cpu:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(op, rp.info.iterations, args.cpu_threads)
