@@ -27,10 +27,13 @@ __global__ __launch_bounds__(kBlock) void k_diagonal(dk_tables t, double *__rest
 {
     extern __shared__ double lds[];
     const int L = (int)t.L, N = (int)t.N, N2 = N + 2, LL = L * L;
-    double *J2 = lds, *X = J2 + LL, *h = X + LL; // J2 = 2 J (exact)
+    // XJ[2 (o L + q) + is]: X[o][q] for is = 0, 2 J[o][q] (exact) for is = 1
+    double *XJ = lds, *h = XJ + 2 * LL;
     uint32_t *bt = (uint32_t *)(h + L);
-    for (int i = threadIdx.x; i < LL; i += blockDim.x) J2[i] = 2 * t.J[i];
-    stage_f64(X, t.X, LL);
+    for (int i = threadIdx.x; i < LL; i += blockDim.x) {
+        XJ[2 * i] = t.X[i];
+        XJ[2 * i + 1] = 2 * t.J[i];
+    }
     stage_f64(h, t.h, L);
     stage_u32(bt, t.binom, (L + 1) * N2);
     __syncthreads();
@@ -47,8 +50,13 @@ __global__ __launch_bounds__(kBlock) void k_diagonal(dk_tables t, double *__rest
         // per q the reference adds 2 is J then (1 - is) X (doci.c:615,619);
         // one of the two is an exact zero that leaves acc unchanged, so a
         // single add of the other gives the same bits
-        const double *Jo = J2 + o * L, *Xo = X + o * L;
-        for (int q = 0; q < L; ++q) acc += ((m >> q) & 1) ? Jo[q] : Xo[q];
+        const double *row = XJ + 2 * o * L;
+        uint64_t mq = m; // bit 0: q occupied
+#pragma unroll 8
+        for (int q = 0; q < L; ++q) {
+            acc += row[2 * q + (int)(mq & 1)];
+            mq >>= 1;
+        }
     }
     D[I] = acc;
 }
