@@ -81,4 +81,44 @@ __device__ __forceinline__ void for_each_hop(uint32_t r, uint64_t m, int L, int 
     }
 }
 
+// The same walk without per-lane branches, for lanes that gather: f(o, t, J,
+// hop) is called for every t != o of every occupied o (L-1 calls per o, the
+// same count in every lane), with hop = (t is empty) and J = rank(I - o + t)
+// when hop (J is meaningless otherwise).  Both binomials of a step are read
+// whether or not t is occupied, so the loop has no divergent branch and the
+// caller's gathers of consecutive steps can be in flight together.
+template <typename F>
+__device__ __forceinline__ void for_each_hop_pred(uint32_t r, uint64_t m, int L, int N,
+                                                  const Binom &C, F &&f)
+{
+    uint64_t rem = m;
+    int h = 0;
+    while (rem) {
+        const int o = __ffsll((unsigned long long)rem) - 1;
+        rem &= rem - 1;
+        const uint32_t base = r + C(L - 1 - o, N - h);
+        int j = h;
+        uint32_t s = 0;
+#pragma unroll 4
+        for (int t = o - 1; t >= 0; --t) {
+            const int occ = (int)((m >> t) & 1);
+            j -= occ;
+            const uint32_t c1 = C(L - 1 - t, N - j), c0 = C(L - 1 - t, N - j - 1);
+            f(o, t, base - c1 + s, occ == 0);
+            s += occ ? c1 - c0 : 0u;
+        }
+        j = h + 1;
+        s = 0;
+#pragma unroll 4
+        for (int t = o + 1; t < L; ++t) {
+            const int occ = (int)((m >> t) & 1);
+            const uint32_t c1 = C(L - 1 - t, N - j + 1), c0 = C(L - 1 - t, N - j);
+            f(o, t, base - c1 - s, occ == 0);
+            s += occ ? c1 - c0 : 0u;
+            j += occ;
+        }
+        ++h;
+    }
+}
+
 } // namespace dk

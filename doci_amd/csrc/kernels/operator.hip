@@ -78,8 +78,10 @@ __global__ __launch_bounds__(kBlock) void k_sigma_lane(dk_tables t, const double
     const dk::Binom C{bt, N2};
     const uint64_t m = dk::unrank(I, L, N, C);
     double acc = (D[I] - shift) * (double)c[I];
-    dk::for_each_hop(I, m, L, N, C, [&](int o, int tt, uint32_t Jr) {
-        acc += K[o * L + tt] * (double)c[Jr];
+    // branch-free walk: a step whose t is occupied reads c[I] and adds 0
+    dk::for_each_hop_pred(I, m, L, N, C, [&](int o, int tt, uint32_t Jr, bool hop) {
+        const double v = K[o * L + tt] * (double)c[hop ? Jr : I];
+        acc += hop ? v : 0.0;
     });
     sigma[I] = (T)acc;
 }
