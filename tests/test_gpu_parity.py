@@ -78,6 +78,42 @@ def test_blocked_kernels_bounds_then_parity(name, fcidump_dir, syn24, tmp_path_f
         op.set_kernel("star")
 
 
+@pytest.mark.parametrize("name", ["co_631g_1", "syn"])
+def test_star_passes_and_limits(name, fcidump_dir, syn24):
+    """Every selectable star pass (0 column, 1 per-star vector, 2 per-star
+    FP64 matrix cores, 3 FP32 column pass on the matrix cores) and rebuilt
+    star tables with other limits: bounds-checked first, then the FP64
+    sigma to the 1e-12 bar and the FP32-storage sigma to 1e-6 of the lane
+    kernel's."""
+    print(f"[case] star passes and limits {name}", flush=True)
+    path = str(syn24) if name == "syn" else str(fcidump_dir / f"{name}.FCIDUMP")
+    with da.Operator.from_fcidump(path) as op:
+        c = da.gen_vector(op.n, 7)
+        op.set_kernel("lane")
+        s_lane = op.sigma(c)
+        s_lane32 = _sigma_fp32(op, c)
+
+        def check(tag):
+            print(f"[case]   {tag} split {op.star_split()}", flush=True)
+            op.set_kernel("star-checked")
+            s = op.sigma(c)
+            s32 = _sigma_fp32(op, c)
+            assert op.check_violations() == 0, tag
+            assert rel_err(s, s_lane) <= SIGMA_RTOL, tag
+            assert rel_err(s32, s_lane32) <= 1e-6, tag
+            op.set_kernel("star")
+            assert rel_err(op.sigma(c), s_lane) <= SIGMA_RTOL, tag
+            assert rel_err(_sigma_fp32(op, c), s_lane32) <= 1e-6, tag
+
+        for p in (1, 2, 3, 0):
+            op.tune("star_pass", p)
+            check(f"star_pass {p}")
+        for key, v in (("star_sgmax", 8), ("star_lds", 40 * 1024), ("star_emax", 2048)):
+            op.tune(key, v)
+            check(f"{key} {v}")
+            op.tune(key, 0)
+
+
 def _sigma_fp32(op, c):
     import ctypes
     n = op.n
