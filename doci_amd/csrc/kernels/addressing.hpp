@@ -121,4 +121,72 @@ __device__ __forceinline__ void for_each_hop_pred(uint32_t r, uint64_t m, int L,
     }
 }
 
+// Hop ranks in separable form, for determinants with N <= NM pairs and
+// L - N <= KM empty orbitals (every array index below is a compile-time
+// constant, so the arrays stay in registers).  With o_h the h-th occupied
+// orbital, t_k the k-th empty one and j = t_k - k the occupied orbitals below
+// t_k, the two O(1) updates above telescope into
+//   t_k < o_h: rank(J) = Pd_h - Td_k,   t_k > o_h: rank(J) = Pu_h - Tu_k,
+//   Pd_h = r + C(L-1-o_h, N-h) + Qd_h,       Td_k = C(L-1-t_k, N-j) + Qd_j,
+//   Pu_h = r + C(L-1-o_h, N-h) + Qu_{h+1},   Tu_k = C(L-1-t_k, N-j+1) + Qu_j,
+// with the prefix sums over the occupied orbitals
+//   Qd_e = sum_{e'<e} C(L-1-o_e', N-e') - C(L-1-o_e', N-e'-1),
+//   Qu_e = sum_{e'<e} C(L-1-o_e', N-e'+1) - C(L-1-o_e', N-e').
+// f(o, t, J) is called for the N(L-N) hops, o ascending, then t ascending.
+template <int NM, int KM, typename F>
+__device__ __forceinline__ void for_each_hop_sep(uint32_t r, uint64_t m, int L, int N,
+                                                 const Binom &C, F &&f)
+{
+    int o[NM];
+    uint32_t Pd[NM], Pu[NM], wd[NM], wu[NM];
+    uint64_t rem = m;
+    uint32_t qd = 0, qu = 0;
+#pragma unroll
+    for (int h = 0; h < NM; ++h) {
+        if (h < N) {
+            const int oh = __ffsll((unsigned long long)rem) - 1;
+            rem &= rem - 1;
+            o[h] = oh;
+            const int x = L - 1 - oh;
+            const uint32_t cm = C(x, N - h - 1), c0 = C(x, N - h), cp = C(x, N - h + 1);
+            Pd[h] = r + c0 + qd;
+            wd[h] = c0 - cm;
+            wu[h] = cp - c0;
+            qd += wd[h];
+            qu += wu[h];
+            Pu[h] = r + c0 + qu;
+        }
+    }
+    const int nk = L - N;
+    int t[KM];
+    uint32_t Td[KM], Tu[KM];
+    uint64_t z = ~m & (L < 64 ? (1ull << L) - 1 : ~0ull);
+#pragma unroll
+    for (int k = 0; k < KM; ++k) {
+        if (k < nk) {
+            const int tk = __ffsll((unsigned long long)z) - 1;
+            z &= z - 1;
+            t[k] = tk;
+            const int j = tk - k, x = L - 1 - tk;
+            uint32_t sd = 0, su = 0;
+#pragma unroll
+            for (int h = 0; h < NM; ++h)
+                if (h < N) {
+                    const bool below = o[h] < tk;
+                    sd += below ? wd[h] : 0u;
+                    su += below ? wu[h] : 0u;
+                }
+            Td[k] = C(x, N - j) + sd;
+            Tu[k] = C(x, N - j + 1) + su;
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < NM; ++h)
+        if (h < N) {
+#pragma unroll
+            for (int k = 0; k < KM; ++k)
+                if (k < nk) f(o[h], t[k], t[k] < o[h] ? Pd[h] - Td[k] : Pu[h] - Tu[k]);
+        }
+}
+
 } // namespace dk

@@ -351,11 +351,17 @@ __global__ __launch_bounds__(kBlock) void k_bsigma_lane(const dk_binst *__restri
     for (uint32_t I = r0 + threadIdx.x; I < r1; I += kBlock) {
         const uint64_t msk = dk::unrank(I, L, N, C);
         double acc = (in.D[I] - shift) * (double)c[I];
-        // branch-free walk: a step whose t is occupied reads c[I] and adds 0
-        dk::for_each_hop_pred(I, msk, L, N, C, [&](int o, int tt, uint32_t Jr, bool hop) {
-            const double v = K[o * L + tt] * (double)c[hop ? Jr : I];
-            acc += hop ? v : 0.0;
-        });
+        if (N <= 12 && L - N <= 16) { // uniform: one instance per workgroup
+            dk::for_each_hop_sep<12, 16>(I, msk, L, N, C, [&](int o, int tt, uint32_t Jr) {
+                acc += K[o * L + tt] * (double)c[Jr];
+            });
+        } else {
+            // branch-free walk: a step whose t is occupied reads c[I] and adds 0
+            dk::for_each_hop_pred(I, msk, L, N, C, [&](int o, int tt, uint32_t Jr, bool hop) {
+                const double v = K[o * L + tt] * (double)c[hop ? Jr : I];
+                acc += hop ? v : 0.0;
+            });
+        }
         sigma[I] = (T)acc;
     }
 }

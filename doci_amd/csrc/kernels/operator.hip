@@ -61,7 +61,8 @@ __global__ __launch_bounds__(kBlock) void k_diagonal(dk_tables t, double *__rest
     D[I] = acc;
 }
 
-template <typename T>
+// kSep: the separable rank form (N <= 12, L - N <= 16), else the branch-free walk
+template <typename T, bool kSep>
 __global__ __launch_bounds__(kBlock) void k_sigma_lane(dk_tables t, const double *__restrict__ D,
                                                        double shift, const T *__restrict__ c,
                                                        T *__restrict__ sigma)
@@ -78,11 +79,17 @@ __global__ __launch_bounds__(kBlock) void k_sigma_lane(dk_tables t, const double
     const dk::Binom C{bt, N2};
     const uint64_t m = dk::unrank(I, L, N, C);
     double acc = (D[I] - shift) * (double)c[I];
-    // branch-free walk: a step whose t is occupied reads c[I] and adds 0
-    dk::for_each_hop_pred(I, m, L, N, C, [&](int o, int tt, uint32_t Jr, bool hop) {
-        const double v = K[o * L + tt] * (double)c[hop ? Jr : I];
-        acc += hop ? v : 0.0;
-    });
+    if (kSep) {
+        dk::for_each_hop_sep<12, 16>(I, m, L, N, C, [&](int o, int tt, uint32_t Jr) {
+            acc += K[o * L + tt] * (double)c[Jr];
+        });
+    } else {
+        // branch-free walk: a step whose t is occupied reads c[I] and adds 0
+        dk::for_each_hop_pred(I, m, L, N, C, [&](int o, int tt, uint32_t Jr, bool hop) {
+            const double v = K[o * L + tt] * (double)c[hop ? Jr : I];
+            acc += hop ? v : 0.0;
+        });
+    }
     sigma[I] = (T)acc;
 }
 
@@ -102,11 +109,18 @@ extern "C" int dk_sigma(const dk_tables *t, const double *D, double shift, const
 {
     const size_t lds = sizeof(double) * t->L * t->L + sizeof(uint32_t) * (t->L + 1) * (t->N + 2);
     const unsigned blocks = (t->n + kBlock - 1) / kBlock;
-    if (prec == 0)
-        hipLaunchKernelGGL(k_sigma_lane<double>, dim3(blocks), dim3(kBlock), lds, s, *t, D,
+    const bool sep = t->N <= 12 && t->L - t->N <= 16;
+    if (prec == 0 && sep)
+        hipLaunchKernelGGL((k_sigma_lane<double, true>), dim3(blocks), dim3(kBlock), lds, s, *t, D,
                            shift, (const double *)c, (double *)sigma);
+    else if (prec == 0)
+        hipLaunchKernelGGL((k_sigma_lane<double, false>), dim3(blocks), dim3(kBlock), lds, s, *t, D,
+                           shift, (const double *)c, (double *)sigma);
+    else if (sep)
+        hipLaunchKernelGGL((k_sigma_lane<float, true>), dim3(blocks), dim3(kBlock), lds, s, *t, D,
+                           shift, (const float *)c, (float *)sigma);
     else
-        hipLaunchKernelGGL(k_sigma_lane<float>, dim3(blocks), dim3(kBlock), lds, s, *t, D,
+        hipLaunchKernelGGL((k_sigma_lane<float, false>), dim3(blocks), dim3(kBlock), lds, s, *t, D,
                            shift, (const float *)c, (float *)sigma);
     return (int)hipGetLastError();
 }
