@@ -152,7 +152,7 @@ int dk_sigma_star_io(const dk_tables *t, const dk_stars *st, const double *D,
 
 typedef struct {             /* constant per instance */
     uint32_t L, N, n;
-    uint32_t tile0, ntiles;  /* its tiles in the tile list */
+    uint32_t tile0, ntiles;  /* its tiles; tile k writes partial-sum row tile0 + k */
     uint32_t tile_rows;      /* rows per tile (ntiles = ceil(n / tile_rows)) */
     uint32_t lane_sigma;     /* 1: sigma by the batched lane kernel */
     uint32_t pad;

@@ -89,6 +89,7 @@ struct doci_batch {
 };
 
 #define ALIGN256(x) (((size_t)(x) + 255) & ~(size_t)255)
+#define DH_XCDS 8 /* XCDs (each with its own L2) of an MI355X */
 
 /* DOCI_BATCH_TRACE=1 prints admission / retirement / step timings to stderr. */
 static int trace_on(void)
@@ -596,6 +597,81 @@ static void drop_graph(bgroup *g)
     g->gM = 0;
 }
 
+/* Launch order of the tiles of group p (h_tiles, g->ntile launch positions).
+ * Workgroup w of a launch runs on XCD w % DH_XCDS (round-robin dispatch), and
+ * each XCD has its own L2.  Instances with the star-kernel sigma stream their
+ * vectors: their tiles come first, in order, spread over every XCD.  Each
+ * lane-sigma instance gathers at random from its own vectors: all its tiles
+ * go to one XCD, so those vectors are cached in one L2 only.  Instances are
+ * dealt to the XCDs largest first, each to the least loaded, and position
+ * base + DH_XCDS q + x takes the q-th tile of XCD x.  Padding positions carry
+ * tile index 0xffffffff, which every tile kernel skips; partial sums are
+ * indexed by tile0 + tile, not by position. */
+static int xcd_order(doci_batch *b, bgroup *g, int p)
+{
+    uint32_t pos = 0;
+    for (int s = 0; s < b->cap; ++s) { /* star-sigma instances, in order */
+        const int ri = b->slot_rec[s];
+        if (ri < 0 || b->rec[ri].prec != p || b->rec[ri].bi.lane_sigma) continue;
+        for (uint32_t k = 0; k < b->rec[ri].bi.ntiles; ++k) {
+            g->h_tiles[2 * pos] = (uint32_t)s;
+            g->h_tiles[2 * pos + 1] = k;
+            ++pos;
+        }
+    }
+    int *xs = malloc(sizeof(int) * (size_t)b->cap), *ord = malloc(sizeof(int) * (size_t)b->cap);
+    if (!xs || !ord) {
+        free(xs);
+        free(ord);
+        return doci_fail(DOCI_E_NOMEM, "batch lists");
+    }
+    int nl = 0, pad_slot = -1;
+    for (int s = 0; s < b->cap; ++s) {
+        const int ri = b->slot_rec[s];
+        if (ri < 0 || b->rec[ri].prec != p) continue;
+        pad_slot = s;
+        if (b->rec[ri].bi.lane_sigma) ord[nl++] = s;
+    }
+    /* largest first (insertion sort: a few hundred instances at most per rebuild) */
+    for (int i = 1; i < nl; ++i) {
+        const int v = ord[i];
+        const uint32_t tv = b->rec[b->slot_rec[v]].bi.ntiles;
+        int j = i;
+        while (j > 0 && b->rec[b->slot_rec[ord[j - 1]]].bi.ntiles < tv) {
+            ord[j] = ord[j - 1];
+            --j;
+        }
+        ord[j] = v;
+    }
+    uint32_t load[DH_XCDS] = {0}, maxload = 0;
+    for (int i = 0; i < nl; ++i) {
+        int x = 0;
+        for (int y = 1; y < DH_XCDS; ++y)
+            if (load[y] < load[x]) x = y;
+        xs[ord[i]] = x;
+        load[x] += b->rec[b->slot_rec[ord[i]]].bi.ntiles;
+        if (load[x] > maxload) maxload = load[x];
+    }
+    const uint32_t base = nl ? (pos + DH_XCDS - 1) / DH_XCDS * DH_XCDS : pos;
+    for (; pos < base + DH_XCDS * maxload; ++pos) { /* padding, overwritten below where used */
+        g->h_tiles[2 * pos] = (uint32_t)(pad_slot < 0 ? 0 : pad_slot);
+        g->h_tiles[2 * pos + 1] = 0xffffffffu;
+    }
+    uint32_t next[DH_XCDS] = {0};
+    for (int i = 0; i < nl; ++i) {
+        const int s = ord[i], x = xs[s];
+        for (uint32_t k = 0; k < b->rec[b->slot_rec[s]].bi.ntiles; ++k) {
+            const uint32_t q = base + DH_XCDS * next[x]++ + (uint32_t)x;
+            g->h_tiles[2 * q] = (uint32_t)s;
+            g->h_tiles[2 * q + 1] = k;
+        }
+    }
+    free(xs);
+    free(ord);
+    g->ntile = pos;
+    return DOCI_OK;
+}
+
 /* Instance lists and tile lists of both precision groups. */
 static int rebuild(doci_batch *b)
 {
@@ -613,10 +689,20 @@ static int rebuild(doci_batch *b)
             const uint32_t rr = rows_for(b->rec[ri].n, rows);
             ntile += (b->rec[ri].n + rr - 1) / rr;
         }
+        /* launch positions: the tiles, plus padding of the XCD order below
+         * (at most 7 + 8 x the tiles of the largest lane-sigma instance) */
+        size_t tmax = 0;
+        for (int s = 0; s < b->cap; ++s) {
+            const int ri = b->slot_rec[s];
+            if (ri < 0 || b->rec[ri].prec != p) continue;
+            const uint32_t rr = rows_for(b->rec[ri].n, rows);
+            if ((b->rec[ri].n + rr - 1) / rr > tmax) tmax = (b->rec[ri].n + rr - 1) / rr;
+        }
+        const size_t nlaunch = ntile + 7 + DH_XCDS * tmax;
         int rc = grow((void **)&g->h_list, (void **)&g->d_list, &g->list_cap, (size_t)cnt + 1,
                       sizeof(uint32_t));
         if (rc == DOCI_OK)
-            rc = grow((void **)&g->h_tiles, (void **)&g->d_tiles, &g->tiles_cap, 2 * ntile + 2,
+            rc = grow((void **)&g->h_tiles, (void **)&g->d_tiles, &g->tiles_cap, 2 * nlaunch + 2,
                       sizeof(uint32_t));
         if (rc == DOCI_OK) rc = grow(NULL, (void **)&g->d_part, &g->part_cap, ntile * DK_BPV + 1, sizeof(double));
         if (rc != DOCI_OK) return rc;
@@ -630,21 +716,18 @@ static int rebuild(doci_batch *b)
             r->bi.tile0 = t;
             r->bi.tile_rows = rows_for(r->n, rows);
             r->bi.ntiles = (r->n + r->bi.tile_rows - 1) / r->bi.tile_rows;
-            for (uint32_t k = 0; k < r->bi.ntiles; ++k) {
-                g->h_tiles[2 * (t + k)] = (uint32_t)s;
-                g->h_tiles[2 * (t + k) + 1] = k;
-            }
             t += r->bi.ntiles;
             if (r->bi.lane_sigma && r->L > g->maxL) g->maxL = r->L;
             if (r->bi.lane_sigma && r->N > g->maxN) g->maxN = r->N;
             HIPCHK(hipMemcpyAsync(b->d_inst + s, &r->bi, sizeof r->bi, hipMemcpyHostToDevice, b->stream));
         }
-        g->ntile = t;
+        rc = xcd_order(b, g, p);
+        if (rc != DOCI_OK) return rc;
         g->fresh = 1;
         if (g->n) {
             HIPCHK(hipMemcpyAsync(g->d_list, g->h_list, sizeof(uint32_t) * g->n,
                                   hipMemcpyHostToDevice, b->stream));
-            HIPCHK(hipMemcpyAsync(g->d_tiles, g->h_tiles, sizeof(uint32_t) * 2 * t,
+            HIPCHK(hipMemcpyAsync(g->d_tiles, g->h_tiles, sizeof(uint32_t) * 2 * g->ntile,
                                   hipMemcpyHostToDevice, b->stream));
         }
     }

@@ -247,14 +247,14 @@ __global__ __launch_bounds__(kBlock) void k_bstream(const dk_binst *__restrict__
     const uint32_t ii = tiles[2 * blockIdx.x], ti = tiles[2 * blockIdx.x + 1];
     const dk_binst in = insts[ii];
     const dk_bstate &st = states[ii];
-    if (st.st.status != DK_B_ACTIVE) return;
+    if (st.st.status != DK_B_ACTIVE || ti >= in.ntiles) return; // ti = 0xffffffff: padding
     const int flags = st.st.flags;
     if (OP == DK_BOP_ORTHO2 && !(flags & DK_BF_ORTHO2)) return;
     if (OP == DK_BOP_DEFLATE && !(flags & DK_BF_DEFLATE)) return;
     const int m = st.st.m;
     const uint32_t r0 = ti * in.tile_rows;
     const uint32_t r1 = min(in.n, r0 + in.tile_rows);
-    double *out = part + (size_t)blockIdx.x * DK_BPV;
+    double *out = part + (size_t)(in.tile0 + ti) * DK_BPV; // the instance's rows, in tile order
 
     const T *V[M];
 #pragma unroll
@@ -335,7 +335,7 @@ __global__ __launch_bounds__(kBlock) void k_bsigma_lane(const dk_binst *__restri
     const uint32_t ii = tiles[2 * blockIdx.x], ti = tiles[2 * blockIdx.x + 1];
     const dk_binst in = insts[ii];
     const dk_bstate &st = states[ii];
-    if (st.st.status != DK_B_ACTIVE || !in.lane_sigma) return;
+    if (st.st.status != DK_B_ACTIVE || !in.lane_sigma || ti >= in.ntiles) return;
     const int L = (int)in.L, N = (int)in.N, N2 = N + 2;
     double *K = lds;
     uint32_t *bt = (uint32_t *)(K + L * L);
