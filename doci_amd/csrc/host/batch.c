@@ -64,6 +64,7 @@ struct doci_batch {
     hipStream_t stream;
     int cap;
     dk_binst *d_inst;
+    dk_binst *h_inst; /* pinned mirror of d_inst, uploaded whole by rebuild() */
     dk_bstate *d_state;
     dk_bstat *h_stat;       /* pinned */
     int *slot_rec;          /* device slot -> record, -1 free */
@@ -291,6 +292,8 @@ int doci_batch_create(const doci_batch_opts *o_in, doci_batch **out)
     int rc = DOCI_OK;
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_inst, sizeof(dk_binst) * b->cap);
+    if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_inst, sizeof(dk_binst) * b->cap, 0);
+    if (e == hipSuccess) memset(b->h_inst, 0, sizeof(dk_binst) * b->cap);
     if (e == hipSuccess) e = hipMalloc((void **)&b->d_state, sizeof(dk_bstate) * b->cap);
     if (e == hipSuccess) e = hipMemset(b->d_state, 0, sizeof(dk_bstate) * b->cap);
     if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_stat, sizeof(dk_bstat) * b->cap, 0);
@@ -370,6 +373,7 @@ void doci_batch_free(doci_batch *b)
     arena_free(b);
     for (int g = 0; g < 2; ++g) group_free(&b->grp[g]);
     if (b->d_inst) hipFree(b->d_inst);
+    if (b->h_inst) hipHostFree(b->h_inst);
     if (b->d_state) hipFree(b->d_state);
     if (b->h_stat) hipHostFree(b->h_stat);
     if (b->d_red) hipFree(b->d_red);
@@ -719,7 +723,7 @@ static int rebuild(doci_batch *b)
             t += r->bi.ntiles;
             if (r->bi.lane_sigma && r->L > g->maxL) g->maxL = r->L;
             if (r->bi.lane_sigma && r->N > g->maxN) g->maxN = r->N;
-            HIPCHK(hipMemcpyAsync(b->d_inst + s, &r->bi, sizeof r->bi, hipMemcpyHostToDevice, b->stream));
+            b->h_inst[s] = r->bi;
         }
         rc = xcd_order(b, g, p);
         if (rc != DOCI_OK) return rc;
@@ -731,6 +735,10 @@ static int rebuild(doci_batch *b)
                                   hipMemcpyHostToDevice, b->stream));
         }
     }
+    /* every running instance's descriptor in one copy (slots without an
+     * instance carry stale entries that no launch reads) */
+    HIPCHK(hipMemcpyAsync(b->d_inst, b->h_inst, sizeof(dk_binst) * (size_t)b->cap,
+                          hipMemcpyHostToDevice, b->stream));
     /* the host lists are sources of async copies: they are rewritten only by
      * the next rebuild, which runs after the stream has drained */
     b->dirty = 0;
