@@ -69,7 +69,7 @@ EXPORTS = [
     "doci_op_dim", "doci_op_L", "doci_op_N", "doci_op_diagonal", "doci_op_sigma",
     "doci_op_sigma_dev", "doci_op_stream", "doci_solve_defaults", "doci_solve",
     "doci_write_fcidump", "doci_gen_integrals", "doci_gen_vector", "doci_read_inputfile",
-    "doci_read_fcidump", "doci_free", "doci_dev_copy", "doci_op_set_kernel",
+    "doci_read_fcidump", "doci_integral_tables", "doci_free", "doci_dev_copy", "doci_op_set_kernel",
     "doci_op_slab_split", "doci_op_check_violations", "doci_op_tune", "doci_op_time_sigma",
     "doci_op_star_split", "doci_op_kernel", "doci_batch_defaults", "doci_batch_create", "doci_batch_free",
     "doci_batch_footprint", "doci_batch_add", "doci_batch_step", "doci_batch_run",

@@ -69,7 +69,6 @@ int dh_read_fcidump(const char *path, unsigned *L, unsigned *Na, unsigned *Nb, d
                     double **one, double **two, int *errline);
 
 /* integrals.c */
-void dh_transform_integrals(double *one, double *two, unsigned L);
 uint64_t dh_binom(unsigned k, unsigned n);
 
 /* operator.c */

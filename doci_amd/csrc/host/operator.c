@@ -63,33 +63,69 @@ void doci_op_free(doci_op *op)
     free(op);
 }
 
+/* (ij|kl) as it stands after the reference's in-place eight-fold unpacking
+ * (src/doci.c:640-655), without unpacking the tensor.  The unpacking visits
+ * the tuples with i>=j, i>=k, k>=l in loop order (i, j, k, l ascending) and
+ * copies each non-zero one to its eight index permutations, so every entry
+ * of a permutation orbit ends up with the first non-zero value among the
+ * orbit's loop-visited tuples (a later visited tuple of the same orbit reads
+ * that copy back); an orbit with none keeps its raw entries. */
+static double eri_unpacked(const double *two, size_t L, size_t i, size_t j, size_t k, size_t l)
+{
+    const size_t L2 = L * L, L3 = L2 * L;
+    const size_t pm[8][4] = {{i, j, k, l}, {k, l, i, j}, {j, i, l, k}, {l, k, j, i},
+                             {j, i, k, l}, {l, k, i, j}, {i, j, l, k}, {k, l, j, i}};
+    int best = -1;
+    for (int q = 0; q < 8; ++q) {
+        const size_t a = pm[q][0], b = pm[q][1], c = pm[q][2], d = pm[q][3];
+        if (!(a >= b && a >= c && c >= d)) continue;
+        if (two[a + L * b + L2 * c + L3 * d] == 0) continue;
+        int earlier = best < 0;
+        for (int e = 0; e < 4 && !earlier; ++e) {
+            if (pm[q][e] == pm[best][e]) continue;
+            earlier = pm[q][e] < pm[best][e];
+            break;
+        }
+        if (earlier) best = q;
+    }
+    const size_t *t = best < 0 ? pm[0] : pm[best];
+    return two[t[0] + L * t[1] + L2 * t[2] + L3 * t[3]];
+}
+
+/* The DOCI tables of raw FCIDUMP integrals (one[L^2], two[L^4] in the
+ * reference's layout): K, J, X (src/doci.c:667-683 and the diagonal's
+ * (pp|qq), (pq|qp)), h[p] = h'_pp = h_pp - 1/2 sum_j (pj|jp) summed with j
+ * ascending (src/doci.c:657-664), each bit-identical to the reference's full
+ * transform followed by the lookups, in O(L^2) instead of O(L^4). */
 int dh_build_tables(unsigned L, unsigned N, const double *one_raw, const double *two_raw,
                     double *K, double *J, double *X, double *h, uint32_t *binom)
 {
-    const size_t l = L, l2 = l * l, l3 = l2 * l;
-    double *one = malloc(sizeof(double) * l2), *two = malloc(sizeof(double) * l2 * l2);
-    if (!one || !two) {
-        free(one);
-        free(two);
-        return DOCI_E_NOMEM;
-    }
-    memcpy(one, one_raw, sizeof(double) * l2);
-    memcpy(two, two_raw, sizeof(double) * l2 * l2);
-    dh_transform_integrals(one, two, L);
-    const int N2 = (int)N + 2;
-    for (size_t p = 0; p < l; ++p) {
-        h[p] = one[p * (l + 1)];
+    const size_t l = L;
+    for (size_t p = 0; p < l; ++p)
         for (size_t q = 0; q < l; ++q) {
-            J[p * l + q] = two[p * (l + 1) + q * (l2 + l3)];          /* (pp|qq) */
-            X[p * l + q] = two[p * (1 + l3) + q * (l + l2)];          /* (pq|qp) */
-            K[p * l + q] = p == q ? 0.0 : two[p * (l2 + 1) + q * (l3 + l)]; /* (pq|pq) */
+            J[p * l + q] = eri_unpacked(two_raw, l, p, p, q, q);                   /* (pp|qq) */
+            X[p * l + q] = eri_unpacked(two_raw, l, p, q, q, p);                   /* (pq|qp) */
+            K[p * l + q] = p == q ? 0.0 : eri_unpacked(two_raw, l, p, q, p, q);    /* (pq|pq) */
         }
+    for (size_t p = 0; p < l; ++p) {
+        double acc = 0;
+        for (size_t j = 0; j < l; ++j) acc += X[p * l + j];
+        h[p] = one_raw[p * (l + 1)] - 0.5 * acc;
     }
-    for (size_t x = 0; x <= l; ++x)
-        for (int j = 0; j < N2; ++j) binom[x * N2 + j] = (uint32_t)dh_binom((unsigned)j, (unsigned)x);
-    free(one);
-    free(two);
+    if (binom) {
+        const int N2 = (int)N + 2;
+        for (size_t x = 0; x <= l; ++x)
+            for (int j = 0; j < N2; ++j) binom[x * N2 + j] = (uint32_t)dh_binom((unsigned)j, (unsigned)x);
+    }
     return DOCI_OK;
+}
+
+int doci_integral_tables(unsigned L, const double *one, const double *two, double *K, double *J,
+                         double *X, double *h)
+{
+    if (L == 0 || L > DK_MAX_L) return doci_fail(DOCI_E_ORBITALS, "Too many orbitals (L=%u).", L);
+    if (!one || !two || !K || !J || !X || !h) return doci_fail(DOCI_E_ARG, "null argument");
+    return dh_build_tables(L, 1, one, two, K, J, X, h, NULL);
 }
 
 /* Allocates an operator with empty host tables for (L, N). */

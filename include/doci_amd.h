@@ -192,6 +192,15 @@ int doci_solve(doci_op *op, const doci_solve_opts *o, const double *guess,
 
 /* ---- FCIDUMP reader/writer and synthetic integrals ---------------------- */
 
+/* The DOCI tables of raw integrals one[L*L], two[L^4] in read_fcidump's
+ * layout (host only, no GPU): K[p*L+q] = (pq|pq) with K[p*L+p] = 0,
+ * J[p*L+q] = (pp|qq), X[p*L+q] = (pq|qp), h[p] = h_pp - 1/2 sum_j (pj|jp),
+ * each bit-identical to transform_integrals (src/doci.c:630-665) followed by
+ * init_Viikk (src/doci.c:667-683) and the diagonal's lookups
+ * (src/doci.c:600-620), read in O(L^2) without unpacking two[]. */
+int doci_integral_tables(unsigned L, const double *one, const double *two, double *K, double *J,
+                         double *X, double *h);
+
 /* read_fcidump's parser returning DOCI_E_IO / DOCI_E_FORMAT / DOCI_E_LINE
  * (with *errline) instead of exiting; free the tables with doci_free. */
 int doci_read_fcidump(const char *path, unsigned *L, unsigned *Na, unsigned *Nb,
