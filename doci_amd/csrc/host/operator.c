@@ -112,10 +112,13 @@ int dh_build_tables(unsigned L, unsigned N, const double *one_raw, const double 
         for (size_t j = 0; j < l; ++j) acc += X[p * l + j];
         h[p] = one_raw[p * (l + 1)] - 0.5 * acc;
     }
-    if (binom) {
+    if (binom) { /* C(x, j) by Pascal's rule (exact: every entry used is < 2^31) */
         const int N2 = (int)N + 2;
         for (size_t x = 0; x <= l; ++x)
-            for (int j = 0; j < N2; ++j) binom[x * N2 + j] = (uint32_t)dh_binom((unsigned)j, (unsigned)x);
+            for (int j = 0; j < N2; ++j)
+                binom[x * N2 + j] = j == 0 ? 1u
+                                    : x == 0 ? 0u
+                                             : binom[(x - 1) * N2 + j - 1] + binom[(x - 1) * N2 + j];
     }
     return DOCI_OK;
 }
