@@ -518,14 +518,9 @@ static int admit(doci_batch *b, int ri, int slot)
      * FP64 restart of a mixed-precision instance */
     bi->x = (b->o.keep_vectors || r->want_mixed) ? r->dx : NULL;
     r->prec = r->want_mixed ? 1 : 0;
-    /* the descriptor is staged too: r->bi may move before the copy runs */
-    char *hb = NULL;
-    if (e == hipSuccess) {
-        rc = stage_reserve(b, sizeof *bi, &hb);
-        if (rc != DOCI_OK) return rc;
-        memcpy(hb, bi, sizeof *bi);
-        e = hipMemcpyAsync(b->d_inst + slot, hb, sizeof *bi, hipMemcpyHostToDevice, b->stream);
-    }
+    /* the device copy of the descriptor comes with the rebuild that puts
+     * the instance in the launch lists (admission marks the batch dirty):
+     * no kernel reads d_inst[slot] before that */
     if (e == hipSuccess)
         e = (hipError_t)dk_batch_start(bi, b->d_state + slot, 0, r->prec, r->prec,
                                        r->prec ? b->o.switch_tol : b->o.tol,

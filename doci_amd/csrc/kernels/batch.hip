@@ -684,23 +684,11 @@ __global__ __launch_bounds__(kSmall) void k_bsmall(int phase, const uint32_t *__
 }
 
 // ---- admission -------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bguess(T *__restrict__ t, const double *__restrict__ x,
-                                                   uint32_t n, const int64_t *__restrict__ idx)
+// The start state of an instance's first phase (one thread).
+__device__ void start_state(dk_bstate *st, const double *__restrict__ D, const int64_t *__restrict__ idx,
+                            int shift_min_d, double tol, double otol, int max_its, int keep,
+                            int max_vec, int its0)
 {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    if (x)
-        t[i] = (T)x[i];
-    else
-        t[i] = (T)((int64_t)i == *idx ? 1.0 : 0.0);
-}
-
-__global__ void k_bstart(dk_bstate *st, const double *__restrict__ D, const int64_t *__restrict__ idx,
-                         int shift_min_d, double tol, double otol, int max_its, int keep,
-                         int max_vec, int its0)
-{
-    if (threadIdx.x != 0) return;
     st->st.status = DK_B_ACTIVE;
     st->st.m = 0;
     st->st.its = its0;
@@ -721,6 +709,24 @@ __global__ void k_bstart(dk_bstate *st, const double *__restrict__ D, const int6
     st->theta = 0;
     st->st.energy = st->shift;
     st->sio.active = 0;
+}
+
+// The first correction t (the guess: x, or the unit vector at argmin D) and,
+// from thread 0 of block 0, the start state: one launch per admitted instance.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bguess(T *__restrict__ t, const double *__restrict__ x,
+                                                   uint32_t n, const int64_t *__restrict__ idx,
+                                                   dk_bstate *st, const double *__restrict__ D,
+                                                   int shift_min_d, double tol, double otol,
+                                                   int max_its, int keep, int max_vec, int its0)
+{
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i == 0) start_state(st, D, idx, shift_min_d, tol, otol, max_its, keep, max_vec, its0);
+    if (i >= n) return;
+    if (x)
+        t[i] = (T)x[i];
+    else
+        t[i] = (T)((int64_t)i == *idx ? 1.0 : 0.0);
 }
 
 template <typename T, int M>
@@ -804,13 +810,10 @@ extern "C" int dk_batch_start(const dk_binst *in, dk_bstate *state_i, int from_x
     void *t = in->slots; // slot 0 holds t at admission
     if (prec)
         hipLaunchKernelGGL(k_bguess<float>, g, b, 0, s, (float *)t, from_x ? in->x : nullptr, in->n,
-                           idx);
+                           idx, state_i, in->D, shift_min_d, tol, otol, max_its, keep, max_vec, its0);
     else
         hipLaunchKernelGGL(k_bguess<double>, g, b, 0, s, (double *)t, from_x ? in->x : nullptr,
-                           in->n, idx);
-    e = (int)hipGetLastError();
-    if (e) return e;
-    hipLaunchKernelGGL(k_bstart, dim3(1), dim3(64), 0, s, state_i, in->D, idx, shift_min_d, tol,
-                       otol, max_its, keep, max_vec, its0);
+                           in->n, idx, state_i, in->D, shift_min_d, tol, otol, max_its, keep,
+                           max_vec, its0);
     return (int)hipGetLastError();
 }
