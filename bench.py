@@ -246,6 +246,7 @@ def main() -> int:
         kern = "sigma FP64"
     achieved = bytes_per / avg
     avg64 = rp.info.seconds_sigma / max(1, rp.info.sigma_calls)
+    traffic = measured_traffic(op, "fp32" if "FP32" in kern else "fp64")
     out = {
         "metric": METRIC,
         "value": round(t_mixed, 4),
@@ -272,8 +273,10 @@ def main() -> int:
         "roofline": {"bound": "hbm", "kernel": kern, "achieved": round(achieved / 1e9, 1),
                      "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK, 4),
-                     "traffic": measured_traffic(op, "fp32" if "FP32" in kern else "fp64"),
+                     "traffic": traffic,
                      "traffic_unit": "GB per sigma call (rocprof FETCH_SIZE + WRITE_SIZE, calibrated)",
+                     # the kernels' own HBM rate: measured traffic over the live sigma time
+                     "traffic_GBps": None if traffic is None else round(traffic / avg, 1),
                      "algorithmic_GB": round(bytes_per / 1e9, 3)},
     }
     if args.cgp:
