@@ -9,7 +9,7 @@
  *   orc_read_fcidump, orc_transform_integrals, orc_exchange_table,
  *   orc_diagonal, orc_sigma  -> bit-exact against oracle/_ref (the reference's
  *                               own src/doci.c + src/read_fcidump.c built by
- *                               oracle/build_ref.sh) on the reference's test
+ *                               oracle/Makefile) on the reference's test
  *                               FCIDUMPs and a synthetic L=24 case.
  *   orc_lex_masks, orc_rank  -> brute-force enumeration, closed form.
  *   orc_connections          -> Slater-Condon rule + brute force; the sigma
@@ -19,10 +19,14 @@
  *   orc_jacobi_eig           -> LAPACK dsyev (scipy) eigenvalues.
  *   orc_sygv2                -> LAPACK dsygv (scipy, driver "gv") including
  *                               eigenvector signs.
- *   orc_ray_q                -> final energies vs Davidson / dense eigh; its
- *                               per-iteration trajectory is parity unpinned
- *                               (no reference log exists and the reference's
- *                               CG needs an external LAPACK to build).
+ *   orc_ray_q                -> per-iteration |g|^2 and Rayleigh quotients,
+ *                               iteration counts and energies of the
+ *                               reference's own CG/CGP runs
+ *                               (tests/golden/solvers.json); a planted
+ *                               Fletcher-Reeves or stopping-test mistake
+ *                               (orc_ray_q_planted) fails them.
+ *   orc_davidson (deflation) -> the same fixtures' Davidson runs with
+ *                               DAVIDSON_MAX_VEC 4 and 6.
  */
 #include "doci_oracle.h"
 
@@ -604,6 +608,19 @@ int orc_ray_q(double *vec, double *energy, double tol, int max_its,
               unsigned L, uint64_t n, int threads, double *trace_res,
               double *trace_e, int trace_cap, int *its_out)
 {
+    return orc_ray_q_planted(vec, energy, tol, max_its, precond, K, D, N, L, n, threads,
+                             trace_res, trace_e, trace_cap, its_out, 0);
+}
+
+/* orc_ray_q with a mistake planted on purpose (tests show that the fixture
+ * bars reject it): mistake 1 = Fletcher-Reeves ratio of the unpreconditioned
+ * norms |g|^2 instead of the preconditioned ones (CG.c:92); 2 = stopping test
+ * on the preconditioned norm instead of |g|^2 (CG.c:90); 0 = none. */
+int orc_ray_q_planted(double *vec, double *energy, double tol, int max_its,
+                      int precond, const double *K, const double *D, unsigned N,
+                      unsigned L, uint64_t n, int threads, double *trace_res,
+                      double *trace_e, int trace_cap, int *its_out, int mistake)
+{
     double *x = malloc(sizeof(double) * n), *p = malloc(sizeof(double) * n);
     double *Ax = malloc(sizeof(double) * n), *Ap = malloc(sizeof(double) * n);
     double *g = malloc(sizeof(double) * n), *pinv = NULL;
@@ -619,10 +636,11 @@ int orc_ray_q(double *vec, double *energy, double tol, int max_its,
     if (pinv)
         for (uint64_t k = 0; k < n; ++k) g[k] *= pinv[k];
     double ng = dot(g, g, n), ngp = ng;
+    double nq_prev = normq; /* for mistake 1 */
     int its = 0;
-    while (normq > tol && its < max_its) {
+    while ((mistake == 2 ? ng : normq) > tol && its < max_its) {
         if (its != 0) {
-            const double q = ng / ngp;
+            const double q = mistake == 1 ? normq / nq_prev : ng / ngp;
             for (uint64_t k = 0; k < n; ++k) p[k] = q * p[k] - g[k];
         } else {
             for (uint64_t k = 0; k < n; ++k) p[k] = -g[k];
@@ -638,6 +656,7 @@ int orc_ray_q(double *vec, double *energy, double tol, int max_its,
         }
         rho = lam[0];
         for (uint64_t k = 0; k < n; ++k) g[k] = 2 * (Ax[k] - rho * x[k]);
+        nq_prev = normq;
         normq = dot(g, g, n);
         if (pinv)
             for (uint64_t k = 0; k < n; ++k) g[k] *= pinv[k];

@@ -76,6 +76,10 @@ int orc_ray_q(double *vec, double *energy, double tol, int max_its,
               int precond, const double *K, const double *D, unsigned N,
               unsigned L, uint64_t n, int threads, double *trace_res,
               double *trace_e, int trace_cap, int *its_out);
+int orc_ray_q_planted(double *vec, double *energy, double tol, int max_its,
+              int precond, const double *K, const double *D, unsigned N,
+              unsigned L, uint64_t n, int threads, double *trace_res,
+              double *trace_e, int trace_cap, int *its_out, int mistake);
 
 /* Dense helpers exposed for their own tests. */
 void orc_jacobi_eig(int m, const double *A, int lda, double *w, double *Z,

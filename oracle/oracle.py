@@ -61,6 +61,8 @@ def lib():
                                 _dp, _dp, C.c_uint, C.c_uint, C.c_uint64, C.c_int,
                                 _dp, _dp, C.c_int, C.POINTER(C.c_int)]
         l.orc_ray_q.restype = C.c_int
+        l.orc_ray_q_planted.argtypes = l.orc_ray_q.argtypes + [C.c_int]
+        l.orc_ray_q_planted.restype = C.c_int
         l.orc_jacobi_eig.argtypes = [C.c_int, _dp, C.c_int, _dp, _dp, C.c_int]
         l.orc_sygv2.argtypes = [_dp, _dp, _dp, _dp]
         l.orc_sygv2.restype = C.c_int
@@ -149,16 +151,19 @@ class Problem:
         k = min(its.value, max_its)
         return dict(rc=rc, energy=e.value, x=x, its=its.value, res=tr_r[:k], e=tr_e[:k])
 
-    def ray_q(self, guess, precond: bool, tol=1e-8, max_its=200, threads=0, D=None):
+    def ray_q(self, guess, precond: bool, tol=1e-8, max_its=200, threads=0, D=None,
+              mistake: int = 0):
+        """mistake != 0 plants an error (see orc_ray_q_planted) for tests
+        that show the fixture bars reject it."""
         D = self.diagonal() if D is None else D
         x = np.ascontiguousarray(guess, dtype=np.float64).copy()
         e = C.c_double()
         its = C.c_int()
         tr_r = np.zeros(max_its + 1)
         tr_e = np.zeros(max_its + 1)
-        rc = lib().orc_ray_q(x, C.byref(e), tol, max_its, int(precond), self.K, D, self.N,
-                             self.L, self.n, threads or os.cpu_count(), tr_r, tr_e,
-                             max_its + 1, C.byref(its))
+        rc = lib().orc_ray_q_planted(x, C.byref(e), tol, max_its, int(precond), self.K, D,
+                                     self.N, self.L, self.n, threads or os.cpu_count(), tr_r,
+                                     tr_e, max_its + 1, C.byref(its), int(mistake))
         k = its.value
         return dict(rc=rc, energy=e.value, x=x, its=k, res=tr_r[:k], e=tr_e[:k])
 

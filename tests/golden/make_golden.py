@@ -1,9 +1,11 @@
 """Writes the golden fixtures of this directory from the reference itself.
 
 Run in a container that has the reference sources at /root/reference and the
-reference operator built by oracle/Makefile (oracle/_ref/libdoci_ref.so):
+reference built by oracle/Makefile (oracle/_ref/libdoci_ref.so and, for the
+solver traces, oracle/_ref/libdoci_ref_solvers.so):
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # golden.json, small.npz, inputs
+    python tests/golden/make_golden.py solvers    # solvers.json
 
 Fixtures (data only):
   fcidump/*.FCIDUMP.gz   the reference's test inputs (tests/*.FCIDUMP), gzip'd
@@ -16,6 +18,17 @@ Fixtures (data only):
                          captured logs hold (tests/coHF.out, tests/lifHF.out,
                          tests/corandom.out, tests/lifrandom.out)
   small.npz              full diagonal and sigma for inputs with n <= 1287
+  solvers.json           runs of the reference's own sparse_eigensolve
+                         (src/wrapper_solvers.c -> src/davidson.c / src/CG.c,
+                         serial, -DDAVIDIT, sequential LAPACK/BLAS): per run
+                         the printed trace rows (IT INFO RESIDUE ENERGY) and
+                         summary line, the return code, the energy in full
+                         precision, the matvec count, per matvec call
+                         <in,in> and <in,H in>, and the eigenvector at sampled
+                         indices (sign fixed by its largest entry).  Cases: CG
+                         and CGP on the small inputs and LiF, Davidson with a
+                         small subspace (deflation every few iterations), and
+                         the synthetic L=24 / N=8 (seed 1) with D and CGP.
 """
 from __future__ import annotations
 
@@ -124,5 +137,135 @@ def main():
     print("wrote", os.path.join(HERE, "golden.json"))
 
 
+# ---------------------------------------------------------------------------
+# Runs of the reference's own solvers
+# ---------------------------------------------------------------------------
+
+# (tag, input, solver, tol, max_its, keep, max_vec)
+SOLVER_RUNS = [
+    *[(f"{f}_{s}", f, s, 1e-8, 3000, 2, 30) for f in SMALL for s in ("CG", "CGP")],
+    ("lif_631g_1_CGP", "lif_631g_1", "CGP", 1e-8, 60, 2, 30),
+    ("lif_631g_1_CG", "lif_631g_1", "CG", 1e-8, 40, 2, 30),
+    ("H2O_631G_D_mv4", "H2O_631G", "D", 1e-8, 500, 2, 4),
+    ("lili_321g_D_mv4", "lili_321g", "D", 1e-8, 500, 2, 4),
+    ("lif_631g_1_D_mv6k3", "lif_631g_1", "D", 1e-8, 200, 3, 6),
+    ("syn_L24_N8_s1_D", "syn_L24_N8_s1", "D", 1e-8, 200, 2, 30),
+    ("syn_L24_N8_s1_CGP", "syn_L24_N8_s1", "CGP", 1e-8, 200, 2, 30),
+]
+
+
+class _Capture:
+    """Redirects the C library's stdout (fd 1) into a file for one call."""
+
+    def __init__(self, libc):
+        self.libc = libc
+
+    def __enter__(self):
+        self.f = tempfile.TemporaryFile(mode="w+b")
+        sys.stdout.flush()
+        self.libc.fflush(None)
+        self.saved = os.dup(1)
+        os.dup2(self.f.fileno(), 1)
+        return self
+
+    def __exit__(self, *exc):
+        self.libc.fflush(None)
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+        self.f.seek(0)
+        self.text = self.f.read().decode()
+        self.f.close()
+
+
+def ref_solve(lib, libc, path, solver, tol, max_its, keep, max_vec):
+    import ctypes as C
+    dat = C.c_void_p()
+    lib.fill_doci_data(path.encode(), C.byref(dat))
+    n = lib.doci_get_basis_size(dat)
+    dptr = lib.doci_get_diagonal(dat)
+    D = np.ctypeslib.as_array(dptr, (n,)).copy()
+    x = np.zeros(n)
+    x[int(np.argmin(D))] = 1.0  # executable.c:276-288, first index of the minimum
+    calls = []
+    MV = C.CFUNCTYPE(None, C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_void_p)
+
+    @MV
+    def matvec(vin, vout, vdat):
+        lib.doci_matvec(vin, vout, vdat)
+        a = np.ctypeslib.as_array(vin, (n,))
+        b = np.ctypeslib.as_array(vout, (n,))
+        calls.append([float(a @ a), float(a @ b)])
+
+    e = C.c_double()
+    with _Capture(libc) as cap:
+        rc = lib.sparse_eigensolve(x.ctypes.data, n, C.byref(e), C.cast(matvec, C.c_void_p),
+                                   C.cast(dptr, C.c_void_p), None, tol, max_its,
+                                   solver.encode(), keep, max_vec, dat)
+    lib.doci_cleanup_data(C.byref(dat))
+    rows, summary = [], None
+    for line in cap.text.splitlines():
+        m = re.match(r"^(\d+)\t(-?\d+)\t([-+.eE\d]+|-?nan|-?inf)\t([-+.\d]+|-?nan)$", line)
+        if m:
+            rows.append([int(m.group(1)), int(m.group(2)), float(m.group(3)), float(m.group(4))])
+        elif "converge" in line:
+            summary = line
+    k = int(np.argmax(np.abs(x)))
+    xs = x * np.sign(x[k])
+    idx = np.unique(np.concatenate([[k], np.argsort(-np.abs(x))[:32],
+                                    np.random.default_rng(5).integers(0, n, 64)]))
+    return dict(n=n, rc=rc, energy=e.value, matvecs=len(calls), calls=calls, trace=rows,
+                summary=summary, x_idx=[int(i) for i in idx], x_val=[float(v) for v in xs[idx]])
+
+
+def solvers_main():
+    import ctypes as C
+    os.environ["MKL_THREADING_LAYER"] = "SEQUENTIAL"
+    os.environ["MKL_NUM_THREADS"] = "1"
+    path = os.path.join(ROOT, "oracle", "_ref", "libdoci_ref_solvers.so")
+    if not os.path.exists(path):
+        raise SystemExit("oracle/_ref/libdoci_ref_solvers.so missing: run make -C oracle ref")
+    lib = C.CDLL(path)
+    libc = C.CDLL(None)
+    libc.fflush.argtypes = [C.c_void_p]
+    lib.fill_doci_data.argtypes = [C.c_char_p, C.POINTER(C.c_void_p)]
+    lib.doci_get_basis_size.argtypes = [C.c_void_p]
+    lib.doci_get_diagonal.argtypes = [C.c_void_p]
+    lib.doci_get_diagonal.restype = C.POINTER(C.c_double)
+    lib.doci_cleanup_data.argtypes = [C.POINTER(C.c_void_p)]
+    lib.doci_matvec.argtypes = [C.c_void_p, C.c_void_p, C.c_void_p]
+    lib.sparse_eigensolve.argtypes = [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.c_void_p,
+                                      C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.c_char_p,
+                                      C.c_int, C.c_int, C.c_void_p]
+    out = {"note": "reference sparse_eigensolve, serial, -DDAVIDIT, sequential MKL",
+           "runs": {}}
+    only = set(sys.argv[2:])
+    with tempfile.TemporaryDirectory() as td:
+        syn = os.path.join(td, "syn_L24_N8_s1.FCIDUMP")
+        da.gen_fcidump(syn, 24, 8, 1)
+        for tag, inp, solver, tol, max_its, keep, max_vec in SOLVER_RUNS:
+            if only and tag not in only:
+                continue
+            p = syn if inp.startswith("syn") else os.path.join(REF, "tests", inp + ".FCIDUMP")
+            t = __import__("time").perf_counter()
+            r = ref_solve(lib, libc, p, solver, tol, max_its, keep, max_vec)
+            r.update(input=inp, solver=solver, tol=tol, max_its=max_its, keep=keep,
+                     max_vec=max_vec)
+            out["runs"][tag] = r
+            print(f"{tag}: rc={r['rc']} E={r['energy']:.12f} matvecs={r['matvecs']} "
+                  f"({__import__('time').perf_counter() - t:.1f} s) {r['summary']}", flush=True)
+    dst = os.path.join(HERE, "solvers.json")
+    if only and os.path.exists(dst):
+        with open(dst) as fh:
+            old = json.load(fh)
+        old["runs"].update(out["runs"])
+        out = old
+    with open(dst, "w") as fh:
+        json.dump(out, fh, separators=(",", ":"))
+    print("wrote", dst)
+
+
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "solvers":
+        solvers_main()
+    else:
+        main()
