@@ -39,7 +39,7 @@ class SolveInfo(C.Structure):
                 ("sigma_calls", C.c_int), ("converged", C.c_int), ("residual", C.c_double),
                 ("energy", C.c_double), ("seconds", C.c_double),
                 ("seconds_sigma", C.c_double), ("sigma_calls_fp32", C.c_int),
-                ("seconds_sigma_fp32", C.c_double)]
+                ("seconds_sigma_fp32", C.c_double), ("d_energy", C.c_double)]
 
 
 class BatchOpts(C.Structure):

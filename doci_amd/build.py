@@ -17,6 +17,7 @@ CSRC = os.path.join(PKG, "csrc")
 OUT = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libdoci_amd.so")
 CLI = os.path.join(PKG, "bin", "doci")
+CALLER = os.path.join(ROOT, "tests", "c", "dropin_caller")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 ARCH = os.environ.get("DOCI_OFFLOAD_ARCH", "gfx950")
 
@@ -66,6 +67,9 @@ def build(verbose: bool = False) -> str:
     _run([gcc, *C_FLAGS, "-o", CLI, os.path.join(CSRC, "cli", "doci_main.c"),
           f"-L{PKG}", "-ldoci_amd", f"-L{ROCM}/lib", "-lamdhip64", "-lm",
           "-Wl,-rpath,$ORIGIN/..", f"-Wl,-rpath,{ROCM}/lib"])
+    # a builder-written caller of the drop-in ABI used by the GPU tests
+    _run([gcc, *C_FLAGS, "-o", CALLER, os.path.join(ROOT, "tests", "c", "dropin_caller.c"),
+          f"-L{PKG}", "-ldoci_amd", "-lm", "-Wl,-rpath,$ORIGIN/../../doci_amd", f"-Wl,-rpath,{ROCM}/lib"])
     return LIB
 
 

@@ -24,7 +24,11 @@
  * Rayleigh quotients.  The restart keeps the FP32 phase's last correction
  * (its preconditioned residual, as an FP64 vector): the first FP64 iteration
  * computes the FP64 sigma of the Ritz vector and then orthogonalises that
- * correction, instead of recomputing the residual of the same vector.
+ * correction, instead of recomputing the residual of the same vector.  The
+ * FP64 phase always runs after an FP32 phase, also when the FP32 residual is
+ * already below tol (switch_tol <= tol, or one iteration past both): an FP32
+ * residual describes the rounded operator, so convergence is only reported
+ * from an FP64 sigma, residual and Rayleigh quotient.
  */
 #include <math.h>
 #include <stdio.h>
@@ -278,6 +282,7 @@ static int dav_phase(dav_run *r, int prec, double shift, double stop_tol, double
             tt = red[1];
             for (int j = 0; j < m; ++j) a[j] = red[2 + j];
         }
+        r->info->d_energy = *energy - theta; /* davidson.c:92-93 */
         *energy = theta;
         if (prec) r->info->iterations_fp32++;
         if (o->trace) printf("%d\t%d\t%e\t%f\n", *r->its, 0, res, theta);
@@ -312,6 +317,17 @@ done:
     return rc;
 }
 
+int dh_diag_min(dh_ctx *c, double *dmin)
+{
+    int64_t idx = 0;
+    int e = dk_argmin(c->d_D, c->n, c->d_red, c->d_idx, c->stream);
+    if (!e) e = hipMemcpyAsync(&idx, c->d_idx, sizeof idx, hipMemcpyDeviceToHost, c->stream);
+    if (!e) e = hipStreamSynchronize(c->stream);
+    if (!e) e = hipMemcpy(dmin, c->d_D + idx, sizeof *dmin, hipMemcpyDeviceToHost);
+    return e ? doci_fail(DOCI_E_HIP, "argmin of the diagonal: %s", hipGetErrorString((hipError_t)e))
+             : DOCI_OK;
+}
+
 int dh_davidson(dh_ctx *c, const doci_solve_opts *o, double *d_x, double *energy,
                 doci_solve_info *info)
 {
@@ -327,18 +343,12 @@ int dh_davidson(dh_ctx *c, const doci_solve_opts *o, double *d_x, double *energy
     HIPCHK(hipEventCreate(&r.ev0));
     HIPCHK(hipEventCreate(&r.ev1));
     *energy = 0;
-    if (o->precision == 1 && c->fp32_ok) {
-        /* shift = min(D): fetched from the device argmin */
+    const int mixed = o->precision == 1 && c->fp32_ok;
+    int ran64 = 0; /* an FP64 phase produced the final residual */
+    if (mixed) {
         double shift = 0;
-        rc = dk_argmin(c->d_D, c->n, c->d_red, c->d_idx, c->stream);
-        int64_t idx = 0;
-        if (!rc) rc = hipMemcpyAsync(&idx, c->d_idx, sizeof idx, hipMemcpyDeviceToHost, c->stream);
-        if (!rc) rc = hipStreamSynchronize(c->stream);
-        if (!rc) rc = hipMemcpy(&shift, c->d_D + idx, sizeof shift, hipMemcpyDeviceToHost);
-        if (rc) {
-            rc = doci_fail(DOCI_E_HIP, "argmin of the diagonal failed");
-            goto out;
-        }
+        rc = dh_diag_min(c, &shift);
+        if (rc) goto out;
         const double sw = o->switch_tol > o->tol ? o->switch_tol : o->tol;
         /* the FP32 phase's last correction seeds the FP64 phase: its first
          * iteration computes the FP64 sigma of the Ritz vector and goes on
@@ -349,11 +359,17 @@ int dh_davidson(dh_ctx *c, const doci_solve_opts *o, double *d_x, double *energy
         res32 = res;
         t_ok = d_t && res <= sw; /* the phase wrote its last correction */
     }
-    if (res > o->tol || o->precision != 1 || !c->fp32_ok) {
+    {
+        /* FP64 phase (the whole solve when not mixed).  After an FP32 phase
+         * that stopped at switch_tol it goes on with the handed-over
+         * correction; after one that already reached tol it starts from the
+         * Ritz vector alone, whose FP64 residual its first iteration checks. */
         const int seeded = t_ok && res32 > o->tol;
-        if (its < o->max_its || its == 0)
+        if (its < o->max_its || its == 0) {
             rc = dav_phase(&r, 0, 0.0, o->tol, d_x, seeded ? d_t : NULL, res32, NULL, energy, &res);
-        if (rc) goto out;
+            if (rc) goto out;
+            ran64 = 1;
+        }
     }
     rc = its >= o->max_its;
 out:
@@ -366,7 +382,7 @@ out:
     info->iterations = its;
     info->residual = res;
     info->energy = *energy;
-    info->converged = res <= o->tol;
+    info->converged = ran64 && res <= o->tol;
     info->seconds = dh_now() - t0;
     return rc;
 }

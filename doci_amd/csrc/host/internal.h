@@ -114,6 +114,10 @@ int dh_ray_q(dh_ctx *c, const doci_solve_opts *o, int precond, double *d_x, doub
 
 double dh_now(void);
 
+/* min(D) of the context's diagonal (device argmin, then one value copied
+ * back): the shift that keeps FP32-stored (H - shift) v entries small. */
+int dh_diag_min(dh_ctx *c, double *dmin);
+
 /* slabs.c */
 int dh_slabs_build(doci_op *op, unsigned emax, unsigned gmax, int xcd_runs);
 void dh_slabs_free(doci_op *op);

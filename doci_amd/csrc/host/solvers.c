@@ -203,11 +203,15 @@ int davidson(double *result, double *energy, int max_vectors, int keep_deflate,
     const int rc = solve_dropin(result, basis_size, energy, matvec, diagonal, &o, vdat, &info);
     if (rc < 0) return dropin_fail(rc);
     if (o.trace) {
-        if (info.iterations <= max_its)
-            printf("Davidson converged in %d iterations with %e residue.\n", info.iterations,
-                   info.residual);
+        /* davidson.c:101-105: the loop counter ends one past max_its when
+         * the limit stops the loop (info.iterations is that counter) */
+        const int its = info.iterations;
+        if (its <= max_its)
+            printf("Davidson converged in %d iterations with %e residue and d_energy %e.\n", its,
+                   info.residual, info.d_energy);
         else
-            printf("Davidson didn't converge within %d iterations!\n", max_its);
+            printf("Davidson didn't converge within %d iterations! d_energy is %e.\n", max_its,
+                   info.d_energy);
         printf("elapsed time : %lf sec\n", info.seconds);
         printf("average time per matvec : %lf sec\n",
                info.iterations ? info.seconds / info.iterations : 0.0);

@@ -163,8 +163,10 @@ typedef struct {
     int max_its;           /* 200 */
     int davidson_keep;     /* 2 */
     int davidson_max_vec;  /* 30 */
-    int precision;         /* 0 = FP64, 1 = mixed (FP32 storage + FP64 finish) */
-    double switch_tol;     /* mixed: residual at which FP64 takes over (1e-5) */
+    int precision;         /* 0 = FP64, 1 = mixed: FP32 storage of the subspace (Davidson)
+                              or of x, p, Ax, Ap, g (CG) with FP64 sums and dots, then an
+                              FP64 finish (FP64 sigma, residual and Rayleigh quotient) */
+    double switch_tol;     /* mixed Davidson: residual at which FP64 takes over (1e-5) */
     int trace;             /* print "its INFO residue energy" lines */
 } doci_solve_opts;
 
@@ -179,6 +181,8 @@ typedef struct {
     double seconds_sigma;  /* of which in sigma kernels (event-timed) */
     int sigma_calls_fp32;  /* sigma launches on FP32 vectors */
     double seconds_sigma_fp32; /* their event-timed total */
+    double d_energy;       /* energy change of the last iteration (the reference's
+                              "d_energy" of its summary line) */
 } doci_solve_info;
 
 void doci_solve_defaults(doci_solve_opts *o);

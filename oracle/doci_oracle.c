@@ -231,15 +231,16 @@ static uint64_t unrank(uint64_t r, unsigned N, unsigned L)
 /* Diagonal (src/doci.c:550-628), exact addition order                       */
 /* ------------------------------------------------------------------------ */
 
-void orc_diagonal(const double *one, const double *two, double core,
-                  unsigned Nu, unsigned Lu, uint64_t n, double *D, int threads)
+/* D at rank `r` (or at ranks[r] when ranks != NULL). */
+static void diagonal_at(const double *one, const double *two, double core, unsigned Nu,
+                        unsigned Lu, int64_t nrow, const uint64_t *ranks, double *D, int threads)
 {
     const size_t L = Lu, L2 = L * L, L3 = L2 * L;
     const int N = (int)Nu;
 #pragma omp parallel for schedule(static) num_threads(threads > 0 ? threads : 1)
-    for (int64_t r = 0; r < (int64_t)n; ++r) {
+    for (int64_t r = 0; r < nrow; ++r) {
         int occ[66];
-        occ_of(unrank((uint64_t)r, Nu, Lu), Lu, occ);
+        occ_of(unrank(ranks ? ranks[r] : (uint64_t)r, Nu, Lu), Lu, occ);
         occ[N] = (int)L;
         double t = core;
         for (int i = N - 1; i >= 0; --i) {
@@ -258,6 +259,18 @@ void orc_diagonal(const double *one, const double *two, double core,
         }
         D[r] = t;
     }
+}
+
+void orc_diagonal(const double *one, const double *two, double core,
+                  unsigned Nu, unsigned Lu, uint64_t n, double *D, int threads)
+{
+    diagonal_at(one, two, core, Nu, Lu, (int64_t)n, NULL, D, threads);
+}
+
+void orc_diagonal_rows(const double *one, const double *two, double core, unsigned N,
+                       unsigned L, const uint64_t *ranks, int64_t count, double *D)
+{
+    diagonal_at(one, two, core, N, L, count, ranks, D, 1);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -44,6 +44,9 @@ uint64_t orc_rank(uint64_t mask, unsigned N, unsigned L);
  * with the same order, so the result does not depend on the thread count. */
 void orc_diagonal(const double *one, const double *two, double core,
                   unsigned N, unsigned L, uint64_t n, double *D, int threads);
+/* The same diagonal entries at `count` given ranks only (large n). */
+void orc_diagonal_rows(const double *one, const double *two, double core, unsigned N,
+                       unsigned L, const uint64_t *ranks, int64_t count, double *D);
 
 /* src/doci.c:851-978: lower-triangle connections of determinant `rank`, in the
  * reference's enumeration order.  Writes up to cap (rank, K-index) pairs and

@@ -50,6 +50,8 @@ def lib():
         l.orc_rank.argtypes = [C.c_uint64, C.c_uint, C.c_uint]
         l.orc_rank.restype = C.c_uint64
         l.orc_diagonal.argtypes = [_dp, _dp, C.c_double, C.c_uint, C.c_uint, C.c_uint64, _dp, C.c_int]
+        l.orc_diagonal_rows.argtypes = [_dp, _dp, C.c_double, C.c_uint, C.c_uint, _up,
+                                        C.c_int64, _dp]
         l.orc_connections.argtypes = [C.c_uint64, C.c_uint, C.c_uint, _up, _u32p, C.c_int]
         l.orc_connections.restype = C.c_int
         l.orc_sigma.argtypes = [_dp, _dp, C.c_uint, C.c_uint, C.c_uint64, _dp, _dp, C.c_int]
@@ -111,6 +113,13 @@ class Problem:
         out = np.zeros(self.n)
         lib().orc_diagonal(self.one, self.two, self.core, self.N, self.L, self.n, out,
                            threads or os.cpu_count())
+        return out
+
+    def diagonal_rows(self, ranks) -> np.ndarray:
+        """Diagonal entries at the given ranks only (same bits as diagonal())."""
+        r = np.ascontiguousarray(ranks, dtype=np.uint64)
+        out = np.zeros(r.size)
+        lib().orc_diagonal_rows(self.one, self.two, self.core, self.N, self.L, r, r.size, out)
         return out
 
     def sigma(self, c: np.ndarray, D: np.ndarray | None = None, threads: int = 1) -> np.ndarray:

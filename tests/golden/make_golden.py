@@ -10,6 +10,7 @@ solver traces, oracle/_ref/libdoci_ref_solvers.so):
 Fixtures (data only):
   fcidump/*.FCIDUMP.gz   the reference's test inputs (tests/*.FCIDUMP), gzip'd
   DOCI_input.in          the reference's template input file
+  logs/*.out             the reference's captured logs (tests/*.out)
   golden.json            per input: the reference operator's diagonal and
                          sigma(seeded vector) as SHA-256 of the raw FP64 bytes
                          plus sampled entries (exact, as uint64 bit patterns);
@@ -116,6 +117,10 @@ def main():
         with open(src, "rb") as fi, open(dst, "wb") as raw, \
                 gzip.GzipFile(fileobj=raw, mode="wb", compresslevel=9, mtime=0) as fo:
             shutil.copyfileobj(fi, fo)
+    # the reference's captured logs (expected outputs of its tests)
+    os.makedirs(os.path.join(HERE, "logs"), exist_ok=True)
+    for k in ["coHF", "lifHF", "corandom", "lifrandom"]:
+        shutil.copyfile(os.path.join(REF, "tests", k + ".out"), os.path.join(HERE, "logs", k + ".out"))
     dst = os.path.join(HERE, "DOCI_input.in")
     if os.path.exists(dst):
         os.chmod(dst, 0o644)

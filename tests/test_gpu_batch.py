@@ -190,3 +190,36 @@ def test_batch_large_instances_use_star_kernel():
                 assert r.status == 2 and r.residual <= 1e-8, (prec, rows, c)
                 assert abs(r.energy - ref.energy) <= 1e-9, (prec, rows, c, r.energy - ref.energy)
                 assert abs(abs(x @ ref.x) - 1) <= 1e-8, (prec, rows, c)
+
+
+def test_batch_alone_equals_in_mix_star_instance():
+    """Continuous batching with a star-kernel instance (2^20 determinants or
+    more): alone and admitted into a running mix of lane- and star-sigma
+    instances, FP64 and mixed.  The star pass sums a centre's hops with LDS
+    atomics, so the order of a few FP64 additions varies from run to run
+    (about 1e-16 relative per sigma): same iteration count, energy within
+    1e-11 Eh, eigenvector within 1e-9 (max abs) and overlap 1 - 1e-12."""
+    print("[case] batch alone vs in mix, star instance", flush=True)
+    probe = (23, 10, 71)
+    assert da.binom(10, 23) >= 1 << 20
+    for prec in ("fp64", "mixed"):
+        with da.Batch(tile_rows=1024) as b:
+            i = b.add_synthetic(*probe, precision=prec)
+            b.run()
+            alone, xa = b.result(i, want_x=True)
+        with da.Batch(tile_rows=1024) as b:
+            first = [b.add_synthetic(L, N, s) for L, N, s in CASES[:3]] + \
+                [b.add_synthetic(24, 9, 72)]
+            b.step()
+            i = b.add_synthetic(*probe, precision=prec)  # joins a running mix
+            b.step()
+            late = [b.add_synthetic(L, N, s, precision="mixed") for L, N, s in CASES[5:]]
+            b.run()
+            mixr, xm = b.result(i, want_x=True)
+            for j in first + late:
+                assert b.result(j)[0].status == 2
+        assert alone.status == 2 and mixr.status == 2
+        assert mixr.iterations == alone.iterations, prec
+        assert abs(mixr.energy - alone.energy) <= 1e-11, (prec, mixr.energy - alone.energy)
+        assert np.abs(xa - xm).max() <= 1e-9, prec
+        assert abs(abs(xa @ xm) - 1) <= 1e-12, prec

@@ -153,3 +153,24 @@ def test_cli_rejects_bad_solver_and_missing_input(tmp_path):
     r = subprocess.run([da.CLI_PATH, "-m", "1", str(tmp_path / "bad.in")], capture_output=True,
                        text=True)
     assert r.returncode == 1 and "Only DOCI works" in r.stderr
+
+
+def test_reference_main_links_against_library():
+    """The reference's own main() (executable.c + macros.c) links against
+    libdoci_amd.so in place of its operator and solver sources: every symbol
+    it binds from doci.h, wrapper_solvers.h and read_fcidump.h resolves in the
+    library.  Built only where the reference sources exist, and never run
+    (tests/c/dropin_caller makes the same calls on the GPU)."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "doci_main_on_amd")
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "refmain"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    if not os.path.exists(exe):
+        pytest.skip("reference sources absent: nothing to link")
+    und = subprocess.run(["nm", "-D", "--undefined-only", exe], capture_output=True,
+                         text=True, check=True).stdout.split()
+    lib = subprocess.run(["nm", "-D", "--defined-only", da.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout.split()
+    for sym in ("fill_doci_data", "doci_get_diagonal", "doci_get_basis_size",
+                "doci_cleanup_data", "doci_matvec", "sparse_eigensolve"):
+        assert sym in und and sym in lib, sym
