@@ -67,7 +67,7 @@ EXPORTS = [
     "doci_cleanup_data", "doci_matvec", "sparse_eigensolve", "davidson", "ray_q",
     "doci_last_error", "doci_op_from_fcidump", "doci_op_from_integrals", "doci_op_free",
     "doci_op_dim", "doci_op_L", "doci_op_N", "doci_op_diagonal", "doci_op_sigma",
-    "doci_op_sigma_dev", "doci_op_stream", "doci_solve_defaults", "doci_solve",
+    "doci_op_sigma_dev", "doci_op_sigma_dev_noalloc", "doci_op_stream", "doci_solve_defaults", "doci_solve",
     "doci_write_fcidump", "doci_gen_integrals", "doci_gen_vector", "doci_read_inputfile",
     "doci_read_fcidump", "doci_integral_tables", "doci_free", "doci_dev_copy", "doci_op_set_kernel",
     "doci_op_slab_split", "doci_op_check_violations", "doci_op_tune", "doci_op_time_sigma",
@@ -99,6 +99,7 @@ def lib():
     l.doci_op_diagonal.argtypes = [vp, _dp]
     l.doci_op_sigma.argtypes = [vp, _dp, _dp]
     l.doci_op_sigma_dev.argtypes = [vp, vp, vp, C.c_int, C.c_double]
+    l.doci_op_sigma_dev_noalloc.argtypes = [vp, vp, vp, C.c_int, C.c_double]
     l.doci_op_stream.argtypes = [vp]
     l.doci_op_stream.restype = vp
     l.doci_solve_defaults.argtypes = [C.POINTER(SolveOpts)]

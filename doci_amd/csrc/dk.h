@@ -133,6 +133,12 @@ typedef struct {
     int32_t pad;
 } dk_sigma_io;
 /* dk_sigma_star with c, sigma and shift taken from *io. */
+/* The star sigma's slot buffer for callers that cannot allocate: a
+ * file-scope device array of dk_static_slot_doubles() FP64 entries, one per
+ * device; dk_static_slot_buffer gives the current device's copy's address
+ * (looked up on the first call per device). */
+size_t dk_static_slot_doubles(void);
+int dk_static_slot_buffer(void **addr);
 int dk_sigma_star_io(const dk_tables *t, const dk_stars *st, const double *D,
                      const dk_sigma_io *io, void *Z, int prec, hipStream_t s);
 

@@ -130,6 +130,9 @@ int dh_stars_build(doci_op *op, unsigned lds_budget, unsigned sgmax, unsigned ma
 void dh_stars_free(doci_op *op);
 void dh_stars_limits(const doci_op *op, unsigned *lds_budget, unsigned *sgmax, unsigned *main_emax);
 int dh_stars_sigma(doci_op *op, const void *in, void *out, int prec, double shift, int checked);
+/* The star sigma with the slot buffer in the library's static device array
+ * (no allocation; the star tables must exist). */
+int dh_stars_sigma_static(doci_op *op, const void *in, void *out, int prec, double shift);
 int64_t dh_stars_violations(doci_op *op);
 int dh_stars_set_pass(doci_op *op, int pass);
 /* The star tables of shape (dt->L, dt->N) with default limits (shared by

@@ -329,6 +329,14 @@ int doci_op_sigma_dev(doci_op *op, const void *in, void *out, int prec, double s
     return DOCI_OK;
 }
 
+int doci_op_sigma_dev_noalloc(doci_op *op, const void *in, void *out, int prec, double shift)
+{
+    if (!dh_is_op(op)) return doci_fail(DOCI_E_ARG, "not an operator");
+    if (!in || !out || in == out) return doci_fail(DOCI_E_ARG, "sigma needs distinct buffers");
+    if (prec != 0 && prec != 1) return doci_fail(DOCI_E_ARG, "prec must be 0 (FP64) or 1 (FP32)");
+    return dh_stars_sigma_static(op, in, out, prec, shift);
+}
+
 int doci_op_set_kernel(doci_op *op, int kind)
 {
     if (kind < 0 || kind > 4) return doci_fail(DOCI_E_ARG, "kernel kind must be 0..4");

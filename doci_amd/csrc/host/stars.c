@@ -503,6 +503,25 @@ int dh_stars_sigma(doci_op *op, const void *in, void *out, int prec, double shif
     return DOCI_OK;
 }
 
+int dh_stars_sigma_static(doci_op *op, const void *in, void *out, int prec, double shift)
+{
+    dh_stars *s = op->stars;
+    if (!s)
+        return doci_fail(DOCI_E_ARG, "star tables not built yet (doci_op_set_kernel(op, 0) or a "
+                                     "first doci_op_sigma_dev builds them; this entry point "
+                                     "allocates nothing)");
+    const size_t need = s->sh->zelems ? s->sh->zelems : 1, cap = dk_static_slot_doubles();
+    if (need > cap)
+        return doci_fail(DOCI_E_ARG, "sigma slot buffer of %zu FP64 entries exceeds the static "
+                                     "buffer of %zu (L=%u, N=%u)", need, cap, op->L, op->N);
+    void *Z = NULL;
+    DKCHK(dk_static_slot_buffer(&Z));
+    dk_stars d = s->sh->d;
+    d.pass = (uint32_t)s->pass;
+    DKCHK(dk_sigma_star(&op->dt, &d, op->d_D, shift, in, out, Z, prec, NULL, op->stream));
+    return DOCI_OK;
+}
+
 int dh_stars_set_pass(doci_op *op, int pass)
 {
     if (!op->stars) {

@@ -19,6 +19,7 @@
 // Main pass.  sigma = (D - shift) c + hops inside the slab (own slab staged in
 // LDS) + the slab's Z slots (coalesced streams).
 #include <hip/hip_runtime.h>
+#include <pthread.h>
 
 #include "../dk.h"
 #include "addressing.hpp"
@@ -1155,6 +1156,45 @@ extern "C" int dk_sigma_star(const dk_tables *t, const dk_stars *st, const doubl
                                      bad, nullptr, s)
                : launch<float, false>(t, st, D, shift, (const float *)c, (float *)sigma, (float *)Z,
                                       bad, nullptr, s);
+}
+
+// ---- the slot buffer of the entry point that allocates nothing ----------
+// doci_op_sigma_dev_noalloc runs this file's star sigma with its slot buffer
+// in `scratch`, a file-scope device array instead of device memory the
+// library allocates: every GPU the library runs on holds one copy (64 MiB)
+// until the process ends.  In host code `scratch` names a host-side stand-in,
+// not a device address, so the address of each device's copy is looked up
+// once per device (hipGetSymbolAddress, that device current) and only that
+// address is passed to the kernels.  The column star pass writes every slot
+// the main pass reads, so the array needs no clearing between calls; calls
+// that overlap on one GPU share it and race.
+constexpr size_t kStaticSlotDoubles = (size_t)8 << 20;
+static __device__ __align__(256) double scratch[kStaticSlotDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+void *g_scratch_addr[kMaxDevices];
+pthread_mutex_t g_scratch_mu = PTHREAD_MUTEX_INITIALIZER;
+} // namespace
+
+extern "C" size_t dk_static_slot_doubles(void) { return kStaticSlotDoubles; }
+
+extern "C" int dk_static_slot_buffer(void **addr)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return (int)e;
+    if (dev < 0 || dev >= kMaxDevices) return (int)hipErrorInvalidDevice;
+    pthread_mutex_lock(&g_scratch_mu);
+    if (!g_scratch_addr[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e == hipSuccess && p) g_scratch_addr[dev] = p;
+        else if (e == hipSuccess) e = hipErrorInvalidSymbol;
+    }
+    *addr = g_scratch_addr[dev];
+    pthread_mutex_unlock(&g_scratch_mu);
+    return (int)e;
 }
 
 extern "C" int dk_sigma_star_io(const dk_tables *t, const dk_stars *st, const double *D,

@@ -112,6 +112,18 @@ int doci_op_sigma(doci_op *op, const double *in, double *out);
 /* Device-buffer sigma on the operator's stream.  prec 0: FP64 vectors;
  * prec 1: FP32 vectors with FP64 sums.  out = (H - shift) in. */
 int doci_op_sigma_dev(doci_op *op, const void *in, void *out, int prec, double shift);
+/* doci_op_sigma_dev without any allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  It
+ * runs the star-kernel sigma (kernel kind 0) with its slot buffer in a
+ * static device array of the library, 8,388,608 FP64 entries (64 MiB) per
+ * GPU, held until the process ends; problems whose slot buffer is larger
+ * (doci_op_star_split's zslots, e.g. L=24 / N=8 and the reference's test
+ * inputs fit, L=34 / N=10 does not) are rejected with DOCI_E_ARG, as is an
+ * operator whose star tables are not built yet (doci_op_set_kernel(op, 0) or
+ * a first doci_op_sigma_dev builds them).  Every call on one GPU shares that
+ * array: calls that overlap on one GPU race. */
+int doci_op_sigma_dev_noalloc(doci_op *op, const void *in, void *out, int prec, double shift);
 /* The operator's HIP stream (hipStream_t) and device. */
 void *doci_op_stream(doci_op *op);
 
