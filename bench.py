@@ -29,29 +29,32 @@ def log(msg: str) -> None:
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def cpu_baseline(op, plain_its: int, threads: int) -> dict:
-    """The oracle's OpenMP path (the reference's lower-triangle walk with
-    thread-private accumulators, MGS Davidson) on the host, on a bounded
-    sample: two Davidson iterations at full size, extrapolated by the GPU's
-    FP64 iteration count."""
-    import numpy as np
+def cpu_baseline(plain_its: int, threads: int, budget_s: float) -> dict:
+    """The oracle's OpenMP restatement of the reference's solve (lower-triangle
+    sigma with thread-private accumulators, MGS Davidson) on the host cores,
+    run as a child process (oracle/cpu_baseline.py) with OMP_NUM_THREADS = the
+    physical cores (lscpu sockets x cores per socket) unless --cpu-threads
+    says otherwise, OMP_PROC_BIND=close, OMP_PLACES=cores: the full solve when
+    it fits the budget, else three iterations extrapolated (flagged)."""
+    import subprocess
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as O
-    core_one_two = __import__("doci_amd").gen_integrals(L, SEED)
-    import tempfile
-    with tempfile.TemporaryDirectory() as td:
-        p = os.path.join(td, "b.FCIDUMP")
-        __import__("doci_amd").write_fcidump(p, L, N, *core_one_two)
-        P = O.Problem(p)
-    D = op.diagonal()  # bit-identical to the oracle's (checked by the tests)
-    g = O.hf_guess(D)
-    t0 = time.perf_counter()
-    r = P.davidson(g, tol=1e-8, max_its=2, D=D, threads=threads)
-    t = time.perf_counter() - t0
-    per_it = t / 2
-    return {"value": round(per_it * plain_its, 3), "unit": "s", "cores": threads, "kind": "port",
-            "sample": f"2 Davidson iterations of the oracle at L=34/N=10 ({t:.1f} s, "
-                      f"{per_it:.2f} s/iteration) x {plain_its} iterations of the FP64 solve"}
+    from cpu_baseline import cpu_info
+    info = cpu_info()
+    t = threads or info["physical_cores"]
+    env = dict(os.environ, OMP_NUM_THREADS=str(t), OMP_PROC_BIND="close", OMP_PLACES="cores")
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "oracle", "cpu_baseline.py"), str(L),
+                        str(N), str(SEED), str(plain_its), str(budget_s)], env=env,
+                       capture_output=True, text=True, timeout=budget_s * 3 + 600)
+    if r.returncode != 0:
+        raise RuntimeError("cpu baseline failed:\n" + r.stderr[-2000:])
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    log(f"cpu baseline: {res['value']} s on {t} threads ({info['model']}, "
+        f"{info['physical_cores']} physical cores), extrapolated={res['extrapolated']}")
+    return {"value": res["value"], "unit": "s", "cores": t, "kind": "port",
+            "extrapolated": res["extrapolated"], "sample": res["sample"],
+            "cpu_model": info["model"], "physical_cores": info["physical_cores"],
+            "omp": "OMP_PROC_BIND=close OMP_PLACES=cores",
+            "iteration_s": res["iteration_s"], "sigma_s": res["sigma_s"]}
 
 
 def measured_traffic(op, prec: str):
@@ -174,7 +177,10 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=0,
+                    help="threads of the CPU baseline (0: the physical cores)")
+    ap.add_argument("--cpu-budget", type=float, default=150.0,
+                    help="seconds the CPU baseline may spend on a full solve")
     ap.add_argument("--calib", action="store_true",
                     help="first run a 1 GiB device copy (byte-counter calibration)")
     ap.add_argument("--cgp", type=int, default=1, help="1: also solve the L=34 problem with CGP")
@@ -292,7 +298,7 @@ def main() -> int:
             f"|dE| vs Davidson {abs(rc.energy - rp.energy):.1e}")
     if not args.no_cpu:
         log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(op, rp.info.iterations, args.cpu_threads)
+        out["cpu_baseline"] = cpu_baseline(rp.info.iterations, args.cpu_threads, args.cpu_budget)
     op.close()
     if args.batch:
         out["batched"] = batched_leg(da, args.batch)

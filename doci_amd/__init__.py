@@ -58,7 +58,8 @@ class BatchResult(C.Structure):
 class InputFile(C.Structure):
     _fields_ = [("fcidump", C.c_char * 255), ("solver", C.c_char * 20), ("max_its", C.c_int),
                 ("tol", C.c_double), ("davidson_keep", C.c_int),
-                ("davidson_max_vec", C.c_int), ("hf_init", C.c_int), ("precision", C.c_int)]
+                ("davidson_max_vec", C.c_int), ("hf_init", C.c_int), ("precision", C.c_int),
+                ("eigvec", C.c_char * 255)]
 
 
 # Every symbol include/doci_amd.h declares (checked by the CPU test suite).
@@ -68,7 +69,7 @@ EXPORTS = [
     "doci_last_error", "doci_op_from_fcidump", "doci_op_from_integrals", "doci_op_free",
     "doci_op_dim", "doci_op_L", "doci_op_N", "doci_op_diagonal", "doci_op_sigma",
     "doci_op_sigma_dev", "doci_op_sigma_dev_noalloc", "doci_op_stream", "doci_solve_defaults", "doci_solve",
-    "doci_write_fcidump", "doci_gen_integrals", "doci_gen_vector", "doci_read_inputfile",
+    "doci_write_fcidump", "doci_write_vector", "doci_read_vector", "doci_gen_integrals", "doci_gen_vector", "doci_read_inputfile",
     "doci_read_fcidump", "doci_integral_tables", "doci_free", "doci_dev_copy", "doci_op_set_kernel",
     "doci_op_slab_split", "doci_op_check_violations", "doci_op_tune", "doci_op_time_sigma",
     "doci_op_star_split", "doci_op_kernel", "doci_batch_defaults", "doci_batch_create", "doci_batch_free",
@@ -109,6 +110,9 @@ def lib():
     l.doci_gen_integrals.argtypes = [C.c_uint, C.c_uint64, C.POINTER(C.c_double), _dp, _dp]
     l.doci_gen_vector.argtypes = [C.c_int64, C.c_uint64, _dp]
     l.doci_read_inputfile.argtypes = [C.c_char_p, C.POINTER(InputFile)]
+    l.doci_write_vector.argtypes = [C.c_char_p, C.c_uint, C.c_uint, C.c_int64, _dp]
+    l.doci_read_vector.argtypes = [C.c_char_p, C.POINTER(C.c_uint), C.POINTER(C.c_uint),
+                                   C.POINTER(C.c_int64), C.POINTER(C.POINTER(C.c_double))]
     l.doci_read_fcidump.argtypes = [C.c_char_p, C.POINTER(C.c_uint), C.POINTER(C.c_uint),
                                     C.POINTER(C.c_uint), C.POINTER(C.c_double),
                                     C.POINTER(C.POINTER(C.c_double)),
@@ -208,7 +212,24 @@ def read_inputfile(path: str) -> dict:
     _check(lib().doci_read_inputfile(path.encode(), C.byref(s)), "doci_read_inputfile")
     return dict(fcidump=s.fcidump.decode(), solver=s.solver.decode(), max_its=s.max_its,
                 tol=s.tol, davidson_keep=s.davidson_keep,
-                davidson_max_vec=s.davidson_max_vec, hf_init=s.hf_init, precision=s.precision)
+                davidson_max_vec=s.davidson_max_vec, hf_init=s.hf_init, precision=s.precision,
+                eigvec=s.eigvec.decode())
+
+
+def read_vector(path: str):
+    """(L, N, x) from an eigenvector file written by doci_write_vector."""
+    L, N, n = C.c_uint(), C.c_uint(), C.c_int64()
+    p = C.POINTER(C.c_double)()
+    _check(lib().doci_read_vector(path.encode(), C.byref(L), C.byref(N), C.byref(n), C.byref(p)),
+           "doci_read_vector")
+    x = np.ctypeslib.as_array(p, (n.value,)).copy()
+    lib().doci_free(C.cast(p, C.c_void_p))
+    return L.value, N.value, x
+
+
+def write_vector(path: str, L: int, N: int, x) -> None:
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    _check(lib().doci_write_vector(path.encode(), L, N, x.size, x), "doci_write_vector")
 
 
 def binom(k: int, n: int) -> int:

@@ -33,7 +33,10 @@ static char doc[] =
     "\n"
     "HF_INIT          = 1: Hartree-Fock initial guess, 0: random. (default: 1)\n"
     "\n"
-    "DOCI_PRECISION   = FP64 or MIXED (FP32 storage, FP64 finish). (default: FP64)\n";
+    "DOCI_PRECISION   = FP64 or MIXED (FP32 storage, FP64 finish). (default: FP64)\n"
+    "\n"
+    "EIGENVECTOR_FILE = path: the converged eigenvector is written there\n"
+    "                   (header line, then FP64 values in determinant order).\n";
 
 static char args_doc[] = "INPUTFILE";
 
@@ -143,6 +146,13 @@ int main(int argc, char *argv[])
     double energy = 0;
     const int info = sparse_eigensolve(x, n, &energy, doci_matvec, diag, NULL, in.tol, in.max_its,
                                        in.solver, in.davidson_keep, in.davidson_max_vec, dat);
+    if (in.eigvec[0]) { /* EIGENVECTOR_FILE: a key of this library */
+        const int rc = doci_write_vector(in.eigvec, doci_op_L(dat), doci_op_N(dat), n, x);
+        if (rc != DOCI_OK) {
+            fprintf(stderr, "%s\n", doci_last_error());
+            exit(EXIT_FAILURE);
+        }
+    }
     gettimeofday(&t1, NULL);
     const double el = (double)(t1.tv_sec - t0.tv_sec) + 1e-6 * (double)(t1.tv_usec - t0.tv_usec);
     printf("\t\t ** EXIT CODE SOLVER : %d\n"

@@ -2,8 +2,10 @@
  *
  * Each line is tried against every key with the same scanf patterns, so
  * whitespace around '=' is optional, '#' lines match nothing and an empty
- * value keeps the default.  One key of this library is added
- * (DOCI_PRECISION = FP64 | MIXED); the reference's parser ignores it. */
+ * value keeps the default.  Two keys of this library are added
+ * (DOCI_PRECISION = FP64 | MIXED, and EIGENVECTOR_FILE = path: where the
+ * CLI writes the converged eigenvector); the reference's parser ignores
+ * them. */
 #include <stdio.h>
 #include <string.h>
 
@@ -30,6 +32,7 @@ int doci_read_inputfile(const char *path, doci_input *in)
         sscanf(line, " DAVIDSON_KEEP = %d ", &in->davidson_keep);
         sscanf(line, " DAVIDSON_MAX_VEC = %d ", &in->davidson_max_vec);
         sscanf(line, " HF_INIT = %d ", &in->hf_init);
+        sscanf(line, " EIGENVECTOR_FILE = %254s ", in->eigvec);
         if (sscanf(line, " DOCI_PRECISION = %31s ", prec) == 1)
             in->precision = !strcmp(prec, "MIXED") || !strcmp(prec, "mixed");
     }

@@ -250,10 +250,20 @@ typedef struct {
     int davidson_max_vec;
     int hf_init;
     int precision;         /* DOCI_PRECISION = MIXED -> 1 (a key of this library) */
+    char eigvec[255];      /* EIGENVECTOR_FILE = path (a key of this library): the CLI
+                              writes the eigenvector there with doci_write_vector */
 } doci_input;
 
 /* Reads the reference's key = value input file with its defaults. */
 int doci_read_inputfile(const char *path, doci_input *in);
+
+/* Eigenvector file (no counterpart in the reference, which keeps the vector
+ * in memory): a text header line "DOCI_VECTOR L N n\n" followed by n FP64
+ * values, little-endian binary, in the determinant order of the basis
+ * (lexicographic order of the sorted occupied orbitals, last fastest). */
+int doci_write_vector(const char *path, unsigned L, unsigned N, int64_t n, const double *x);
+/* Reads such a file; *x (malloc'd, free with doci_free) receives n values. */
+int doci_read_vector(const char *path, unsigned *L, unsigned *N, int64_t *n, double **x);
 
 /* ---- many independent solves on one GPU ----------------------------------
  * No counterpart in the reference (one solve per process there).  A batch

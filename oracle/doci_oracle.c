@@ -531,6 +531,26 @@ int orc_davidson(double *result, double *energy, int max_vec, int keep,
                  unsigned L, uint64_t n, int max_its, int threads,
                  double *trace_res, double *trace_e, int trace_cap, int *its_out)
 {
+    return orc_davidson_timed(result, energy, max_vec, keep, tol, K, D, N, L, n, max_its, threads,
+                              trace_res, trace_e, NULL, trace_cap, its_out);
+}
+
+static double wall_now(void)
+{
+#ifdef _OPENMP
+    return omp_get_wtime();
+#else
+    return 0.0;
+#endif
+}
+
+int orc_davidson_timed(double *result, double *energy, int max_vec, int keep,
+                       double tol, const double *K, const double *D, unsigned N,
+                       unsigned L, uint64_t n, int max_its, int threads,
+                       double *trace_res, double *trace_e, double *trace_t,
+                       int trace_cap, int *its_out)
+{
+    const double t_start = wall_now();
     const size_t mv = (size_t)max_vec;
     double *S = calloc(mv * mv, sizeof(double));
     double *Y = calloc(mv * mv, sizeof(double));
@@ -598,6 +618,7 @@ int orc_davidson(double *result, double *energy, int max_vec, int keep,
         if (trace_res && its - 1 < trace_cap) {
             trace_res[its - 1] = res;
             trace_e[its - 1] = th;
+            if (trace_t) trace_t[its - 1] = wall_now() - t_start;
         }
         /* create_new_vec_t, davidson.c:214-227 */
 #pragma omp parallel for schedule(static)

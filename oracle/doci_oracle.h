@@ -70,6 +70,13 @@ int orc_davidson(double *result, double *energy, int max_vec, int keep,
                  unsigned L, uint64_t n, int max_its, int threads,
                  double *trace_res, double *trace_e, int trace_cap,
                  int *its_out);
+/* orc_davidson that also records, per iteration, the wall seconds since the
+ * solve started (trace_t, may be NULL). */
+int orc_davidson_timed(double *result, double *energy, int max_vec, int keep,
+                       double tol, const double *K, const double *D, unsigned N,
+                       unsigned L, uint64_t n, int max_its, int threads,
+                       double *trace_res, double *trace_e, double *trace_t,
+                       int trace_cap, int *its_out);
 
 /* src/CG.c:13-164: Rayleigh-quotient CG, preconditioned by 1/D when
  * precond != 0.  The 2x2 generalized eigensolve follows LAPACK dsygv's

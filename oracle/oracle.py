@@ -59,6 +59,9 @@ def lib():
                                    _dp, _dp, C.c_uint, C.c_uint, C.c_uint64, C.c_int, C.c_int,
                                    _dp, _dp, C.c_int, C.POINTER(C.c_int)]
         l.orc_davidson.restype = C.c_int
+        l.orc_davidson_timed.argtypes = l.orc_davidson.argtypes[:14] + [_dp] + \
+            l.orc_davidson.argtypes[14:]
+        l.orc_davidson_timed.restype = C.c_int
         l.orc_ray_q.argtypes = [_dp, C.POINTER(C.c_double), C.c_double, C.c_int, C.c_int,
                                 _dp, _dp, C.c_uint, C.c_uint, C.c_uint64, C.c_int,
                                 _dp, _dp, C.c_int, C.POINTER(C.c_int)]
@@ -154,11 +157,13 @@ class Problem:
         its = C.c_int()
         tr_r = np.zeros(max_its + 1)
         tr_e = np.zeros(max_its + 1)
-        rc = lib().orc_davidson(x, C.byref(e), max_vec, keep, tol, self.K, D, self.N, self.L,
-                                self.n, max_its, threads or os.cpu_count(), tr_r, tr_e,
-                                max_its + 1, C.byref(its))
+        tr_t = np.zeros(max_its + 1)
+        rc = lib().orc_davidson_timed(x, C.byref(e), max_vec, keep, tol, self.K, D, self.N,
+                                      self.L, self.n, max_its, threads or os.cpu_count(), tr_r,
+                                      tr_e, tr_t, max_its + 1, C.byref(its))
         k = min(its.value, max_its)
-        return dict(rc=rc, energy=e.value, x=x, its=its.value, res=tr_r[:k], e=tr_e[:k])
+        return dict(rc=rc, energy=e.value, x=x, its=its.value, res=tr_r[:k], e=tr_e[:k],
+                    t=tr_t[:k])
 
     def ray_q(self, guess, precond: bool, tol=1e-8, max_its=200, threads=0, D=None,
               mistake: int = 0):

@@ -247,3 +247,24 @@ def test_gpu_mixed_solve_L34():
         e = float(x @ hx)
         assert abs(e - rmx.energy) <= 1e-9
         assert np.linalg.norm(hx - e * x) <= 1e-8 * 1.05
+
+
+def test_gpu_cli_writes_eigenvector(fcidump_dir, golden, tmp_path):
+    """EIGENVECTOR_FILE: the CLI writes the converged unit eigenvector in
+    determinant order; its Rayleigh quotient (FP64 sigma) is the printed
+    energy and it equals the library solve's vector up to sign."""
+    print("[case] CLI eigenvector file", flush=True)
+    path = fcidump_dir / "H2O_631G.FCIDUMP"
+    (tmp_path / "in.in").write_text(f"FCIDUMP = {path}\nEIGENVECTOR_FILE = {tmp_path / 'x.vec'}\n")
+    r = subprocess.run([da.CLI_PATH, str(tmp_path / "in.in")], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    e_cli = float([l for l in r.stdout.splitlines() if "DOCI ENERGY" in l][0].split(":")[1])
+    L, N, x = da.read_vector(str(tmp_path / "x.vec"))
+    with da.Operator.from_fcidump(str(path)) as op:
+        assert (L, N, x.size) == (op.L, op.N, op.n)
+        assert abs(np.linalg.norm(x) - 1) <= 1e-10
+        e = float(x @ op.sigma(x))
+        ref = op.solve("D")
+    assert abs(e - e_cli) <= 1e-9 and abs(e - golden["H2O_631G"]["dense_lowest"][0]) <= 1e-9
+    assert abs(abs(x @ ref.x) - 1) <= 1e-10

@@ -127,7 +127,7 @@ def test_inputfile_parser_matches_reference_echo():
     DAVIDSON_KEEP / DAVIDSON_MAX_VEC keep the defaults 2 and 30."""
     d = da.read_inputfile(os.path.join(GOLDEN, "DOCI_input.in"))
     assert d == dict(fcidump="co_631g_1.FCIDUMP", solver="D", max_its=1000, tol=1e-8,
-                     davidson_keep=2, davidson_max_vec=30, hf_init=1, precision=0)
+                     davidson_keep=2, davidson_max_vec=30, hf_init=1, precision=0, eigvec="")
 
 
 def test_inputfile_defaults_and_keys(tmp_path):
@@ -140,7 +140,7 @@ def test_inputfile_defaults_and_keys(tmp_path):
                                    "# SOLVER = CG\nDOCI_PRECISION = MIXED\n")
     d = da.read_inputfile(str(tmp_path / "b.in"))
     assert d == dict(fcidump="y", solver="CGP", max_its=7, tol=1e-6, davidson_keep=3,
-                     davidson_max_vec=12, hf_init=0, precision=1)
+                     davidson_max_vec=12, hf_init=0, precision=1, eigvec="")
 
 
 def test_cli_rejects_bad_solver_and_missing_input(tmp_path):
@@ -174,3 +174,23 @@ def test_reference_main_links_against_library():
     for sym in ("fill_doci_data", "doci_get_diagonal", "doci_get_basis_size",
                 "doci_cleanup_data", "doci_matvec", "sparse_eigensolve"):
         assert sym in und and sym in lib, sym
+
+
+def test_eigenvector_file_round_trip_and_key(tmp_path):
+    """EIGENVECTOR_FILE = path is read from the input file (the reference's
+    parser ignores the key), and doci_write_vector / doci_read_vector round
+    trip a vector bit for bit with its (L, N) header; a damaged file is a
+    format error."""
+    (tmp_path / "in.in").write_text("FCIDUMP = a.FCIDUMP\nEIGENVECTOR_FILE = out.vec\n")
+    assert da.read_inputfile(str(tmp_path / "in.in"))["eigvec"] == "out.vec"
+    assert da.read_inputfile(os.path.join(GOLDEN, "DOCI_input.in"))["eigvec"] == ""
+    x = da.gen_vector(1287, 5)
+    p = str(tmp_path / "x.vec")
+    da.write_vector(p, 13, 5, x)
+    L, N, y = da.read_vector(p)
+    assert (L, N) == (13, 5) and np.array_equal(x.view(np.uint64), y.view(np.uint64))
+    raw = open(p, "rb").read()
+    assert raw.startswith(b"DOCI_VECTOR 13 5 1287\n") and len(raw) == 22 + 8 * 1287
+    open(p, "wb").write(raw[:-8])
+    with pytest.raises(da.DociError):
+        da.read_vector(p)
