@@ -217,6 +217,32 @@ int dk_batch_start(const dk_binst *inst_host, dk_bstate *state_i, int from_x, in
                    int shift_min_d, double tol, double otol, int max_its, int keep, int max_vec,
                    int its0, double *red, int64_t *idx, hipStream_t s);
 
+/* Admission of many instances at once.  Each admitted instance's tables
+ * (the layout of the instance memory: K, J, X (L x L FP64), h (L FP64),
+ * binomials ((L+1) x (N+2) u32), tb bytes) sit in a device staging copy at
+ * src; dk_batch_admit copies them to dst, computes the diagonal D
+ * (bit-identical to dk_diagonal), clears the guess slot t, and per instance
+ * sets t = e_argmin(D) (first index of the minimum) and the start state as
+ * dk_batch_start does: two launches for all of them.  Rows are covered by
+ * tiles of DK_ADMIT_ROWS; the tiles of admitted instance a are
+ * [tile0, tile0 + ntiles) of the partial buffer (2 doubles per tile). */
+#define DK_ADMIT_ROWS 256
+typedef struct {
+    dk_tables src;           /* L, N, n, core and the tables in the staging copy */
+    void *dst;               /* instance table memory (tb bytes, 8-aligned) */
+    uint32_t tb;
+    uint32_t tile0, ntiles;
+    int32_t prec, shift_min_d, max_its, keep, max_vec;
+    double *D;
+    void *t;                 /* guess slot (prec 0: FP64, 1: FP32) */
+    dk_bstate *state;
+    double tol, otol;
+} dk_badmit;
+/* adm: na descriptors, tiles: (admitted index, tile) pairs, part: 2 doubles
+ * per tile; all device memory. */
+int dk_batch_admit(const dk_badmit *adm, uint32_t na, const uint32_t *tiles, uint32_t ntile,
+                   double *part, uint32_t maxL, uint32_t maxN, hipStream_t s);
+
 /* ---- streaming vector kernels (deterministic two-level reductions) -------
  * Vectors are arrays of n elements; `vecs` lists device pointers.  prec as
  * above for the stored vectors; reductions and coefficients are FP64.

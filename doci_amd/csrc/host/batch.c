@@ -82,6 +82,16 @@ struct doci_batch {
     int64_t *d_idx;
     char *h_stage;          /* pinned staging of instance tables (reused every step) */
     size_t stage_cap, stage_used;
+    char *d_stage;          /* its device copy (same offsets) */
+    size_t stage_sent;      /* bytes of h_stage already copied to d_stage */
+    dk_badmit *h_adm;       /* pinned: admissions not yet launched, then their tiles */
+    char *d_adm;            /* device copy of h_adm */
+    size_t adm_cap;         /* bytes of h_adm / d_adm */
+    double *d_apart;        /* 2 doubles per admission tile */
+    size_t apart_cap;       /* tiles */
+    int npend;              /* admissions in h_adm */
+    uint32_t pend_tiles;
+    uint32_t pend_maxL, pend_maxN;
     int inflight;           /* an iteration (and its status copy) is queued */
     void **chunk;           /* device arena chunks */
     int nchunk, chunk_cap;
@@ -380,6 +390,10 @@ void doci_batch_free(doci_batch *b)
     if (b->d_red) hipFree(b->d_red);
     if (b->d_idx) hipFree(b->d_idx);
     if (b->h_stage) hipHostFree(b->h_stage);
+    if (b->d_stage) hipFree(b->d_stage);
+    if (b->h_adm) hipHostFree(b->h_adm);
+    if (b->d_adm) hipFree(b->d_adm);
+    if (b->d_apart) hipFree(b->d_apart);
     if (b->stream) hipStreamSynchronize(b->stream);
     if (b->stream) hipStreamDestroy(b->stream);
     free(b->slot_rec);
@@ -425,25 +439,100 @@ int doci_batch_add(doci_batch *b, unsigned L, unsigned N, double core, const dou
     return DOCI_OK;
 }
 
-/* Room for `bytes` in the pinned staging buffer (growing it after the
- * stream has drained the copies that read it). */
+static int flush_admissions(doci_batch *b);
+
+/* Room for `bytes` in the pinned staging buffer (and its device copy),
+ * launching the pending admissions and waiting for the copies that read the
+ * buffer before it is reused or grown. */
 static int stage_reserve(doci_batch *b, size_t bytes, char **p)
 {
     if (b->stage_used + bytes > b->stage_cap) {
+        const int rc = flush_admissions(b);
+        if (rc != DOCI_OK) return rc;
         HIPCHK(hipStreamSynchronize(b->stream));
-        b->stage_used = 0;
+        b->stage_used = b->stage_sent = 0;
         if (bytes > b->stage_cap) {
             size_t nc = b->stage_cap ? 2 * b->stage_cap : (1u << 20);
             while (nc < bytes) nc *= 2;
             if (b->h_stage) hipHostFree(b->h_stage);
-            b->h_stage = NULL;
+            if (b->d_stage) hipFree(b->d_stage);
+            b->h_stage = b->d_stage = NULL;
             b->stage_cap = 0;
             HIPCHK(hipHostMalloc((void **)&b->h_stage, nc, 0));
+            HIPCHK(hipMalloc((void **)&b->d_stage, nc));
             b->stage_cap = nc;
         }
     }
     *p = b->h_stage + b->stage_used;
     b->stage_used += ALIGN256(bytes);
+    return DOCI_OK;
+}
+
+/* Grows the pinned admission list (after the copies that read it) to hold
+ * `na` descriptors and `ntile` tiles. */
+static int adm_reserve(doci_batch *b, int na, uint32_t ntile)
+{
+    const size_t need = sizeof(dk_badmit) * (size_t)na + 2 * sizeof(uint32_t) * ntile;
+    if (need > b->adm_cap) {
+        HIPCHK(hipStreamSynchronize(b->stream));
+        size_t nc = b->adm_cap ? 2 * b->adm_cap : (64u << 10);
+        while (nc < need) nc *= 2;
+        dk_badmit *h = NULL;
+        HIPCHK(hipHostMalloc((void **)&h, nc, 0));
+        if (b->h_adm) {
+            memcpy(h, b->h_adm, sizeof(dk_badmit) * (size_t)b->npend);
+            hipHostFree(b->h_adm);
+        }
+        b->h_adm = h;
+        if (b->d_adm) hipFree(b->d_adm);
+        b->d_adm = NULL;
+        HIPCHK(hipMalloc((void **)&b->d_adm, nc));
+        b->adm_cap = nc;
+    }
+    if (ntile > b->apart_cap) {
+        HIPCHK(hipStreamSynchronize(b->stream));
+        size_t nc = b->apart_cap ? 2 * b->apart_cap : 4096;
+        while (nc < ntile) nc *= 2;
+        if (b->d_apart) hipFree(b->d_apart);
+        b->d_apart = NULL;
+        HIPCHK(hipMalloc((void **)&b->d_apart, sizeof(double) * 2 * nc));
+        b->apart_cap = nc;
+    }
+    return DOCI_OK;
+}
+
+/* Launches the pending admissions: one copy of the staged tables, one of
+ * the descriptors and tiles, and two kernels for all of them (tables to the
+ * instance memory, diagonal, guess at argmin D, start state). */
+static int flush_admissions(doci_batch *b)
+{
+    if (!b->npend) return DOCI_OK;
+    const int na = b->npend;
+    const uint32_t nt = b->pend_tiles;
+    int rc = adm_reserve(b, na, nt);
+    if (rc != DOCI_OK) return rc;
+    uint32_t *tiles = (uint32_t *)(b->h_adm + na);
+    uint32_t q = 0;
+    for (int a = 0; a < na; ++a)
+        for (uint32_t k = 0; k < b->h_adm[a].ntiles; ++k) {
+            tiles[2 * q] = (uint32_t)a;
+            tiles[2 * q + 1] = k;
+            ++q;
+        }
+    if (b->stage_used > b->stage_sent)
+        HIPCHK(hipMemcpyAsync(b->d_stage + b->stage_sent, b->h_stage + b->stage_sent,
+                              b->stage_used - b->stage_sent, hipMemcpyHostToDevice, b->stream));
+    b->stage_sent = b->stage_used;
+    const size_t ab = sizeof(dk_badmit) * (size_t)na + 2 * sizeof(uint32_t) * nt;
+    HIPCHK(hipMemcpyAsync(b->d_adm, b->h_adm, ab, hipMemcpyHostToDevice, b->stream));
+    DKCHK(dk_batch_admit((const dk_badmit *)b->d_adm, (uint32_t)na,
+                         (const uint32_t *)(b->d_adm + sizeof(dk_badmit) * (size_t)na), nt,
+                         b->d_apart, b->pend_maxL, b->pend_maxN, b->stream));
+    /* the copies read h_adm asynchronously: the next admissions wait for
+     * them (drain or stage_reserve synchronises the stream first) */
+    b->npend = 0;
+    b->pend_tiles = 0;
+    b->pend_maxL = b->pend_maxN = 0;
     return DOCI_OK;
 }
 
@@ -507,8 +596,6 @@ static int admit(doci_batch *b, int ri, int slot)
     r->bytes = mb + r->zbytes;
     r->xbytes = sizeof(double) * (size_t)n;
     b->mem_used += r->bytes + r->xbytes;
-    hipError_t e = hipMemcpyAsync(d, h, tb, hipMemcpyHostToDevice, b->stream);
-    if (e == hipSuccess) e = (hipError_t)dk_diagonal(&dt, D, b->stream);
     bi->binom = db;
     bi->K = dK;
     bi->D = D;
@@ -520,13 +607,33 @@ static int admit(doci_batch *b, int ri, int slot)
     r->prec = r->want_mixed ? 1 : 0;
     /* the device copy of the descriptor comes with the rebuild that puts
      * the instance in the launch lists (admission marks the batch dirty):
-     * no kernel reads d_inst[slot] before that */
-    if (e == hipSuccess)
-        e = (hipError_t)dk_batch_start(bi, b->d_state + slot, 0, r->prec, r->prec,
-                                       r->prec ? b->o.switch_tol : b->o.tol,
-                                       r->prec ? 1e-6 : 1e-12, b->o.max_its, b->o.keep,
-                                       b->o.max_vec, 0, b->d_red, b->d_idx, b->stream);
-    if (e != hipSuccess) return doci_fail(DOCI_E_HIP, "instance start: %s", hipGetErrorString(e));
+     * no kernel reads d_inst[slot] before that.  Tables, diagonal, guess and
+     * start state are made by the step's admission launches
+     * (flush_admissions), from the tables staged here. */
+    rc = adm_reserve(b, b->npend + 1, 0);
+    if (rc != DOCI_OK) return rc;
+    dk_badmit *ad = &b->h_adm[b->npend++];
+    memset(ad, 0, sizeof *ad);
+    const size_t so = (size_t)(h - b->h_stage); /* same offset in the device copy */
+    double *sK = (double *)(b->d_stage + so), *sJ = sK + l2, *sX = sJ + l2, *sh = sX + l2;
+    ad->src = (dk_tables){L, N, n, r->core, (uint32_t *)(sh + l), sK, sJ, sX, sh};
+    ad->dst = d;
+    ad->tb = (uint32_t)tb;
+    ad->tile0 = b->pend_tiles;
+    ad->ntiles = (n + DK_ADMIT_ROWS - 1) / DK_ADMIT_ROWS;
+    ad->prec = r->prec;
+    ad->shift_min_d = r->prec;
+    ad->max_its = b->o.max_its;
+    ad->keep = b->o.keep;
+    ad->max_vec = b->o.max_vec;
+    ad->D = D;
+    ad->t = bi->slots;
+    ad->state = b->d_state + slot;
+    ad->tol = r->prec ? b->o.switch_tol : b->o.tol;
+    ad->otol = r->prec ? 1e-6 : 1e-12;
+    b->pend_tiles += ad->ntiles;
+    if (L > b->pend_maxL) b->pend_maxL = L;
+    if (N > b->pend_maxN) b->pend_maxN = N;
     free(r->tab);
     r->tab = NULL;
     if (trace_on())
@@ -568,7 +675,7 @@ static int admit_queued(doci_batch *b)
         if (rc != DOCI_OK) return rc;
         ++running;
     }
-    return DOCI_OK;
+    return flush_admissions(b);
 }
 
 static int grow(void **h, void **d, size_t *cap, size_t need, size_t elem)
@@ -932,7 +1039,7 @@ static int drain(doci_batch *b)
     if (!b->inflight) return DOCI_OK;
     HIPCHK(hipStreamSynchronize(b->stream));
     b->inflight = 0;
-    b->stage_used = 0; /* every staged copy has completed */
+    b->stage_used = b->stage_sent = 0; /* every staged copy has completed */
     for (int s = 0; s < b->cap; ++s) {
         const int ri = b->slot_rec[s];
         if (ri < 0) continue;

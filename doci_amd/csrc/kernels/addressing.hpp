@@ -189,4 +189,30 @@ __device__ __forceinline__ void for_each_hop_sep(uint32_t r, uint64_t m, int L, 
         }
 }
 
+// Diagonal entry of determinant mask m (src/doci.c:589-624), the reference's
+// exact addition order: start at core, occupied orbitals in descending order
+// add 2 h'_o, then per q ascending 2 is J_oq then (1 - is) X_oq.  One of the
+// two per-q terms is an exact zero that leaves the sum unchanged, so a single
+// add of the other gives the same bits.  XJ[2 (o L + q) + is] holds X_oq for
+// is = 0 and 2 J_oq (exact) for is = 1.
+__device__ __forceinline__ double diag_of(uint64_t m, int L, double core, const double *XJ,
+                                          const double *h)
+{
+    double acc = core;
+    uint64_t rem = m;
+    while (rem) {
+        const int o = 63 - __clzll((long long)rem);
+        rem ^= 1ull << o;
+        acc += 2 * h[o];
+        const double *row = XJ + 2 * o * L;
+        uint64_t mq = m; // bit 0: q occupied
+#pragma unroll 8
+        for (int q = 0; q < L; ++q) {
+            acc += row[2 * q + (int)(mq & 1)];
+            mq >>= 1;
+        }
+    }
+    return acc;
+}
+
 } // namespace dk

@@ -729,6 +729,103 @@ __global__ __launch_bounds__(kBlock) void k_bguess(T *__restrict__ t, const doub
         t[i] = (T)((int64_t)i == *idx ? 1.0 : 0.0);
 }
 
+// ---- admission of many instances: two launches --------------------------
+// First index of the minimum: (value, index) pairs, ties to the lower index.
+__device__ __forceinline__ void amin2(double &v, double &ix, double v2, double ix2)
+{
+    if (v2 < v || (v2 == v && ix2 < ix)) {
+        v = v2;
+        ix = ix2;
+    }
+}
+
+// Per tile of an admitted instance: its tables in LDS (from the staging
+// copy), D for the tile's rows (the reference's exact order), the guess slot
+// cleared, the tile's (min D, first index) in part; tile 0 also copies the
+// tables to the instance memory.
+__global__ __launch_bounds__(kBlock) void k_badmit_diag(const dk_badmit *__restrict__ adm,
+                                                        const uint32_t *__restrict__ tiles,
+                                                        double *__restrict__ part)
+{
+    extern __shared__ double lds[];
+    __shared__ double sv[kBlock], si[kBlock];
+    const uint32_t ai = tiles[2 * blockIdx.x], k = tiles[2 * blockIdx.x + 1];
+    const dk_badmit &A = adm[ai];
+    const dk_tables &t = A.src;
+    const int L = (int)t.L, N = (int)t.N, N2 = N + 2, LL = L * L;
+    double *XJ = lds, *h = XJ + 2 * LL;
+    uint32_t *bt = (uint32_t *)(h + L);
+    for (int i = threadIdx.x; i < LL; i += kBlock) {
+        XJ[2 * i] = t.X[i];
+        XJ[2 * i + 1] = 2 * t.J[i];
+    }
+    for (int i = threadIdx.x; i < L; i += kBlock) h[i] = t.h[i];
+    for (int i = threadIdx.x; i < (L + 1) * N2; i += kBlock) bt[i] = t.binom[i];
+    if (k == 0)
+        for (uint32_t i = threadIdx.x; i < A.tb / 4; i += kBlock)
+            ((uint32_t *)A.dst)[i] = ((const uint32_t *)t.K)[i];
+    __syncthreads();
+    const uint32_t I = k * DK_ADMIT_ROWS + threadIdx.x;
+    double v = INFINITY, ix = 1e300;
+    if (I < t.n) {
+        const dk::Binom C{bt, N2};
+        const double d = dk::diag_of(dk::unrank(I, L, N, C), L, t.core, XJ, h);
+        A.D[I] = d;
+        if (A.prec) ((float *)A.t)[I] = 0.f;
+        else ((double *)A.t)[I] = 0.0;
+        v = d;
+        ix = (double)I;
+    }
+    sv[threadIdx.x] = v;
+    si[threadIdx.x] = ix;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            double a = sv[threadIdx.x], b = si[threadIdx.x];
+            amin2(a, b, sv[threadIdx.x + w], si[threadIdx.x + w]);
+            sv[threadIdx.x] = a;
+            si[threadIdx.x] = b;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * (A.tile0 + k)] = sv[0];
+        part[2 * (A.tile0 + k) + 1] = si[0];
+    }
+}
+
+// Per admitted instance: the argmin over its tiles' partials (a total order,
+// so the merge order does not matter), t[argmin] = 1, the start state.
+__global__ __launch_bounds__(kBlock) void k_badmit_start(const dk_badmit *__restrict__ adm,
+                                                         const double *__restrict__ part)
+{
+    __shared__ double sv[kBlock], si[kBlock];
+    __shared__ int64_t idx;
+    const dk_badmit &A = adm[blockIdx.x];
+    double v = INFINITY, ix = 1e300;
+    for (uint32_t k = threadIdx.x; k < A.ntiles; k += kBlock)
+        amin2(v, ix, part[2 * (A.tile0 + k)], part[2 * (A.tile0 + k) + 1]);
+    sv[threadIdx.x] = v;
+    si[threadIdx.x] = ix;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            double a = sv[threadIdx.x], b = si[threadIdx.x];
+            amin2(a, b, sv[threadIdx.x + w], si[threadIdx.x + w]);
+            sv[threadIdx.x] = a;
+            si[threadIdx.x] = b;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        idx = (int64_t)si[0];
+        if (A.prec) ((float *)A.t)[idx] = 1.f;
+        else ((double *)A.t)[idx] = 1.0;
+        start_state(A.state, A.D, &idx, A.shift_min_d, A.tol, A.otol, A.max_its, A.keep,
+                    A.max_vec, 0);
+    }
+}
+
 template <typename T, int M>
 int stream_launch(int op, const dk_binst *inst, dk_bstate *state, const uint32_t *tiles,
                   uint32_t ntile, double *part, hipStream_t s)
@@ -815,5 +912,17 @@ extern "C" int dk_batch_start(const dk_binst *in, dk_bstate *state_i, int from_x
         hipLaunchKernelGGL(k_bguess<double>, g, b, 0, s, (double *)t, from_x ? in->x : nullptr,
                            in->n, idx, state_i, in->D, shift_min_d, tol, otol, max_its, keep,
                            max_vec, its0);
+    return (int)hipGetLastError();
+}
+
+extern "C" int dk_batch_admit(const dk_badmit *adm, uint32_t na, const uint32_t *tiles,
+                              uint32_t ntile, double *part, uint32_t maxL, uint32_t maxN,
+                              hipStream_t s)
+{
+    if (na == 0) return 0;
+    const size_t lds = sizeof(double) * (2 * maxL * maxL + maxL) +
+                       sizeof(uint32_t) * (maxL + 1) * (maxN + 2);
+    hipLaunchKernelGGL(k_badmit_diag, dim3(ntile), dim3(kBlock), lds, s, adm, tiles, part);
+    hipLaunchKernelGGL(k_badmit_start, dim3(na), dim3(kBlock), 0, s, adm, part);
     return (int)hipGetLastError();
 }

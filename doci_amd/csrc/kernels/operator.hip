@@ -40,25 +40,7 @@ __global__ __launch_bounds__(kBlock) void k_diagonal(dk_tables t, double *__rest
     const uint32_t I = blockIdx.x * kBlock + threadIdx.x;
     if (I >= t.n) return;
     const dk::Binom C{bt, N2};
-    const uint64_t m = dk::unrank(I, L, N, C);
-    double acc = t.core;
-    uint64_t rem = m;
-    while (rem) { // occupied orbitals in descending order (doci.c:597)
-        const int o = 63 - __clzll((long long)rem);
-        rem ^= 1ull << o;
-        acc += 2 * h[o];
-        // per q the reference adds 2 is J then (1 - is) X (doci.c:615,619);
-        // one of the two is an exact zero that leaves acc unchanged, so a
-        // single add of the other gives the same bits
-        const double *row = XJ + 2 * o * L;
-        uint64_t mq = m; // bit 0: q occupied
-#pragma unroll 8
-        for (int q = 0; q < L; ++q) {
-            acc += row[2 * q + (int)(mq & 1)];
-            mq >>= 1;
-        }
-    }
-    D[I] = acc;
+    D[I] = dk::diag_of(dk::unrank(I, L, N, C), L, t.core, XJ, h);
 }
 
 // kSep: the separable rank form (N <= 12, L - N <= 16), else the branch-free walk
