@@ -9,8 +9,10 @@ the device-memory budget (FIFO), runs it in lock step with everything else
 that is running, and retires it when it converges.  Time-to-converge of a
 request = its completion time - its arrival time, on the batch's clock.
 
-`one_at_a_time` times the baseline the stream is compared with: the same
-requests solved one after the other as plain FP64 single-instance solves.
+`one_at_a_time` and `one_at_a_time_queue` time the baseline the stream is
+compared with: the same requests solved one after the other as plain FP64
+single-instance solves (the second as a real-time FIFO server fed by the
+same arrivals).
 """
 from __future__ import annotations
 
@@ -102,6 +104,32 @@ def one_at_a_time(requests) -> dict:
     return dict(service=np.array(times), energy=np.array(energies))
 
 
+def one_at_a_time_queue(requests, arrivals, integrals=None) -> dict:
+    """A one-at-a-time server measured in real time: requests are solved in
+    arrival order as plain FP64 single-instance solves (operator build from
+    the in-memory integrals + Davidson), each starting when it has arrived
+    and the previous one has finished.  Returns per-request time-to-converge
+    (finish - arrival) and service times."""
+    ints = list(integrals) if integrals is not None else [_integrals(q) for q in requests]
+    t0 = time.perf_counter()
+    lat, service, energies = [], [], []
+    for k, (L, N, core, one, two) in enumerate(ints):
+        wait = arrivals[k] - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        ts = time.perf_counter()
+        with Operator.from_integrals(L, N, core, one, two) as op:
+            r = op.solve("D", want_x=False)
+        tf = time.perf_counter()
+        service.append(tf - ts)
+        lat.append(tf - t0 - arrivals[k])
+        energies.append(r.energy)
+    lat = np.array(lat)
+    return dict(latency=lat, service=np.array(service), energy=np.array(energies),
+                p50=float(np.percentile(lat, 50)), p99=float(np.percentile(lat, 99)),
+                makespan=time.perf_counter() - t0)
+
+
 def fifo_latencies(arrivals, service) -> np.ndarray:
     """Time-to-converge of a one-at-a-time FIFO queue with these service
     times: finish_i = max(arrival_i, finish_{i-1}) + service_i."""
@@ -110,3 +138,31 @@ def fifo_latencies(arrivals, service) -> np.ndarray:
         fin = max(a, fin) + s
         out.append(fin - a)
     return np.array(out)
+
+
+def main(argv=None) -> int:
+    """python -m doci_amd.serve [--requests R] [--rate X] [--precision P]:
+    the serving workload's first R requests (doci_amd.workloads) at Poisson
+    rate X times the one-at-a-time FP64 throughput of a sample; prints
+    solves/s and p50/p99 time-to-converge."""
+    import argparse
+    from .workloads import poisson_arrivals, serving_requests
+    ap = argparse.ArgumentParser(prog="python -m doci_amd.serve")
+    ap.add_argument("--requests", type=int, default=500)
+    ap.add_argument("--rate", type=float, default=10.0)
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
+    a = ap.parse_args(argv)
+    reqs = serving_requests(a.requests)
+    sample = reqs[:: max(1, a.requests // 40)]
+    one_at_a_time(sample)  # warm the per-shape tables
+    mu0 = 1.0 / float(one_at_a_time(sample)["service"].mean())
+    res = run_stream(reqs, poisson_arrivals(a.requests, a.rate * mu0), a.precision)
+    print(f"{a.requests} requests at {a.rate} x {mu0:.1f}/s ({a.precision}): "
+          f"{res['solves_per_s']:.1f} solves/s, p50 {res['p50']:.3f} s, p99 {res['p99']:.3f} s, "
+          f"peak running {res['peak_running']}, all converged "
+          f"{bool((res['status'] == 2).all())}", flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
