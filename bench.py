@@ -79,33 +79,38 @@ def batched_leg(da, count: int) -> dict:
     ints = [da.gen_integrals(L, s) for L, N, s in inst]
     out = {"instances": count, "dims": [min(da.binom(N, L) for L, N, _ in inst),
                                         max(da.binom(N, L) for L, N, _ in inst)]}
-    # The tile-size autotuner times the batched kernels once per process for an
-    # instance set and caches the choice; it is warmed here, untimed (one
-    # admission + tuning + first iteration), as the other legs warm their
-    # once-per-shape tables.  Its cost is reported as warmup_s.
-    t = time.perf_counter()
+    # One engine serves every batch, as a long-lived service would: its first
+    # batch (untimed, reported as cold_s) allocates the device arena and the
+    # pinned buffers and runs the tile-size autotuner once, as the other legs
+    # warm their once-per-shape tables.  Each timed batch then covers request
+    # intake (the host tables), admission, the lock-step iterations and
+    # retirement; the median of three batches is reported.
+    energies = {}
     with da.Batch() as b:
+        t = time.perf_counter()
         for k, (L, N, _) in enumerate(inst):
             b.add(L, N, *ints[k])
-        b.step()
-    out["warmup_s"] = round(time.perf_counter() - t, 3)
-    energies = {}
-    for prec in ("fp64", "mixed"):
-        t = time.perf_counter()
-        with da.Batch() as b:
-            ids = [b.add(L, N, *ints[k], precision=prec) for k, (L, N, _) in enumerate(inst)]
-            b.run()
-            t = time.perf_counter() - t
+        b.run()
+        out["cold_s"] = round(time.perf_counter() - t, 3)
+        for prec in ("fp64", "mixed"):
+            runs, it0 = [], b.stats()["iterations"]
+            for rep in range(3):
+                t = time.perf_counter()
+                ids = [b.add(L, N, *ints[k], precision=prec) for k, (L, N, _) in enumerate(inst)]
+                b.run()
+                runs.append(time.perf_counter() - t)
             res = [b.result(i)[0] for i in ids]
             st = b.stats()
-        energies[prec] = np.array([r.energy for r in res])
-        out[prec] = {"seconds": round(t, 3), "solves_per_s": round(count / t, 1),
-                     "lockstep_iterations": st["iterations"], "tile_rows": st["tile_rows"],
-                     "graph_launches": st["graph_launches"],
-                     "all_converged": all(r.status == 2 for r in res),
-                     "max_iterations": max(r.iterations for r in res)}
-        log(f"batched {prec}: {count} solves in {t:.3f} s ({count / t:.1f}/s), "
-            f"{st['iterations']} lock-step iterations, tile rows {st['tile_rows']}")
+            t = float(np.median(runs))
+            energies[prec] = np.array([r.energy for r in res])
+            out[prec] = {"seconds": round(t, 4), "solves_per_s": round(count / t, 1),
+                         "runs_s": [round(x, 4) for x in runs],
+                         "lockstep_iterations": (st["iterations"] - it0) // 3,
+                         "tile_rows": st["tile_rows"],
+                         "all_converged": all(r.status == 2 for r in res),
+                         "max_iterations": max(r.iterations for r in res)}
+            log(f"batched {prec}: {count} solves in {t:.4f} s ({count / t:.1f}/s, median of "
+                f"{[round(x, 4) for x in runs]}), tile rows {st['tile_rows']}")
     t = time.perf_counter()
     e1 = []
     for k, (L, N, _) in enumerate(inst):

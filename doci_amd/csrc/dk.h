@@ -210,20 +210,14 @@ int dk_batch_small(int phase, const uint32_t *ilist, uint32_t ninst, const dk_bi
 /* sigma of every active lane-sigma instance: W[sout] = (H - shift) V[sin]. */
 int dk_batch_sigma_lane(int prec, uint32_t maxL, uint32_t maxN, const dk_binst *inst,
                         dk_bstate *state, const uint32_t *tiles, uint32_t ntile, hipStream_t s);
-/* Admission of instance i (FP64 x, D on device): t = e_argmin(D) (or t = x
- * when from_x), state reset for precision prec; shift = D[argmin] when
- * shift_min_d (else 0).  red: dk_red_scratch_doubles(1) doubles. */
-int dk_batch_start(const dk_binst *inst_host, dk_bstate *state_i, int from_x, int prec,
-                   int shift_min_d, double tol, double otol, int max_its, int keep, int max_vec,
-                   int its0, double *red, int64_t *idx, hipStream_t s);
-
 /* Admission of many instances at once.  Each admitted instance's tables
  * (the layout of the instance memory: K, J, X (L x L FP64), h (L FP64),
  * binomials ((L+1) x (N+2) u32), tb bytes) sit in a device staging copy at
  * src; dk_batch_admit copies them to dst, computes the diagonal D
  * (bit-identical to dk_diagonal), clears the guess slot t, and per instance
- * sets t = e_argmin(D) (first index of the minimum) and the start state as
- * dk_batch_start does: two launches for all of them.  Rows are covered by
+ * sets t = e_argmin(D) (first index of the minimum) and the start state of
+ * its first phase (precision prec, shift = min D when shift_min_d, else 0):
+ * two launches for all of them.  Rows are covered by
  * tiles of DK_ADMIT_ROWS; the tiles of admitted instance a are
  * [tile0, tile0 + ntiles) of the partial buffer (2 doubles per tile). */
 #define DK_ADMIT_ROWS 256
@@ -242,6 +236,21 @@ typedef struct {
  * per tile; all device memory. */
 int dk_batch_admit(const dk_badmit *adm, uint32_t na, const uint32_t *tiles, uint32_t ntile,
                    double *part, uint32_t maxL, uint32_t maxN, hipStream_t s);
+
+/* FP64 restart of many mixed-precision instances at once: the FP32 phase's
+ * Ritz vector x (stored FP64) becomes the guess t of the FP64 phase, and the
+ * start state is that of an FP64 first phase without shift, continuing the
+ * iteration count from its0: one launch for all of them.  tiles: (restart index, tile) pairs of DK_ADMIT_ROWS rows. */
+typedef struct {
+    const double *x;
+    double *t;
+    dk_bstate *state;
+    uint32_t n, its0;
+    int32_t max_its, keep, max_vec, pad;
+    double tol, otol;
+} dk_brestart;
+int dk_batch_restart(const dk_brestart *rs, const uint32_t *tiles, uint32_t ntile,
+                     hipStream_t s);
 
 /* ---- streaming vector kernels (deterministic two-level reductions) -------
  * Vectors are arrays of n elements; `vecs` lists device pointers.  prec as

@@ -78,8 +78,6 @@ struct doci_batch {
     double t0;
     int tile_rows, tuned;
     int iterations, graph_launches;
-    double *d_red;
-    int64_t *d_idx;
     char *h_stage;          /* pinned staging of instance tables (reused every step) */
     size_t stage_cap, stage_used;
     char *d_stage;          /* its device copy (same offsets) */
@@ -92,6 +90,11 @@ struct doci_batch {
     int npend;              /* admissions in h_adm */
     uint32_t pend_tiles;
     uint32_t pend_maxL, pend_maxN;
+    dk_brestart *h_rst;     /* pinned: FP64 restarts of this drain, then their tiles */
+    char *d_rst;            /* device copy of h_rst */
+    size_t rst_cap;         /* bytes of h_rst / d_rst */
+    int nrst;               /* restarts in h_rst */
+    uint32_t rst_tiles;
     int inflight;           /* an iteration (and its status copy) is queued */
     void **chunk;           /* device arena chunks */
     int nchunk, chunk_cap;
@@ -309,8 +312,6 @@ int doci_batch_create(const doci_batch_opts *o_in, doci_batch **out)
     if (e == hipSuccess) e = hipMemset(b->d_state, 0, sizeof(dk_bstate) * b->cap);
     if (e == hipSuccess) e = hipHostMalloc((void **)&b->h_stat, sizeof(dk_bstat) * b->cap, 0);
     if (e == hipSuccess) memset(b->h_stat, 0, sizeof(dk_bstat) * b->cap);
-    if (e == hipSuccess) e = hipMalloc((void **)&b->d_red, sizeof(double) * dk_red_scratch_doubles(4));
-    if (e == hipSuccess) e = hipMalloc((void **)&b->d_idx, sizeof(int64_t));
     if (e != hipSuccess) rc = doci_fail(DOCI_E_HIP, "batch setup: %s", hipGetErrorString(e));
     b->slot_rec = malloc(sizeof(int) * b->cap);
     if (rc == DOCI_OK && !b->slot_rec) rc = doci_fail(DOCI_E_NOMEM, "batch");
@@ -387,13 +388,13 @@ void doci_batch_free(doci_batch *b)
     if (b->h_inst) hipHostFree(b->h_inst);
     if (b->d_state) hipFree(b->d_state);
     if (b->h_stat) hipHostFree(b->h_stat);
-    if (b->d_red) hipFree(b->d_red);
-    if (b->d_idx) hipFree(b->d_idx);
     if (b->h_stage) hipHostFree(b->h_stage);
     if (b->d_stage) hipFree(b->d_stage);
     if (b->h_adm) hipHostFree(b->h_adm);
     if (b->d_adm) hipFree(b->d_adm);
     if (b->d_apart) hipFree(b->d_apart);
+    if (b->h_rst) hipHostFree(b->h_rst);
+    if (b->d_rst) hipFree(b->d_rst);
     if (b->stream) hipStreamSynchronize(b->stream);
     if (b->stream) hipStreamDestroy(b->stream);
     free(b->slot_rec);
@@ -1004,13 +1005,79 @@ static int autotune(doci_batch *b)
     return DOCI_OK;
 }
 
+/* Grows the pinned restart list to `nr` descriptors and `ntile` tiles,
+ * keeping the descriptors recorded so far.  Called from drain, after the
+ * stream has synchronised: the previous list's copy has completed. */
+static int rst_reserve(doci_batch *b, int nr, uint32_t ntile)
+{
+    const size_t need = sizeof(dk_brestart) * (size_t)nr + 2 * sizeof(uint32_t) * ntile;
+    if (need <= b->rst_cap) return DOCI_OK;
+    size_t nc = b->rst_cap ? 2 * b->rst_cap : (64u << 10);
+    while (nc < need) nc *= 2;
+    dk_brestart *h = NULL;
+    HIPCHK(hipHostMalloc((void **)&h, nc, 0));
+    if (b->h_rst) {
+        memcpy(h, b->h_rst, sizeof(dk_brestart) * (size_t)b->nrst);
+        hipHostFree(b->h_rst);
+    }
+    b->h_rst = h;
+    if (b->d_rst) hipFree(b->d_rst);
+    b->d_rst = NULL;
+    HIPCHK(hipMalloc((void **)&b->d_rst, nc));
+    b->rst_cap = nc;
+    return DOCI_OK;
+}
+
+/* The FP32 phase of a mixed-precision instance met the switch tolerance: its
+ * FP64 phase restarts from the Ritz vector (recorded here, launched for all
+ * of the drain's restarts by flush_restarts). */
 static int restart_fp64(doci_batch *b, brec *r, const dk_bstat *st)
 {
+    int rc = rst_reserve(b, b->nrst + 1, 0);
+    if (rc != DOCI_OK) return rc;
     r->res.iterations_fp32 = st->its;
     r->prec = 0;
-    DKCHK(dk_batch_start(&r->bi, b->d_state + r->slot, 1, 0, 0, b->o.tol, 1e-12, b->o.max_its,
-                         b->o.keep, b->o.max_vec, st->its, b->d_red, b->d_idx, b->stream));
+    dk_brestart *q = &b->h_rst[b->nrst++];
+    memset(q, 0, sizeof *q);
+    q->x = r->bi.x;
+    q->t = (double *)r->bi.slots; /* slot 0 holds t at a (re)start */
+    q->state = b->d_state + r->slot;
+    q->n = r->n;
+    q->its0 = (uint32_t)st->its;
+    q->max_its = b->o.max_its;
+    q->keep = b->o.keep;
+    q->max_vec = b->o.max_vec;
+    q->tol = b->o.tol;
+    q->otol = 1e-12;
+    b->rst_tiles += (r->n + DK_ADMIT_ROWS - 1) / DK_ADMIT_ROWS;
     b->dirty = 1;
+    return DOCI_OK;
+}
+
+/* One copy of the descriptors and tiles and one launch for every restart
+ * recorded by this drain. */
+static int flush_restarts(doci_batch *b)
+{
+    if (!b->nrst) return DOCI_OK;
+    const int nr = b->nrst;
+    const uint32_t nt = b->rst_tiles;
+    int rc = rst_reserve(b, nr, nt);
+    if (rc != DOCI_OK) return rc;
+    uint32_t *tiles = (uint32_t *)(b->h_rst + nr);
+    uint32_t q = 0;
+    for (int i = 0; i < nr; ++i)
+        for (uint32_t k = 0; k < (b->h_rst[i].n + DK_ADMIT_ROWS - 1) / DK_ADMIT_ROWS; ++k) {
+            tiles[2 * q] = (uint32_t)i;
+            tiles[2 * q + 1] = k;
+            ++q;
+        }
+    const size_t bytes = sizeof(dk_brestart) * (size_t)nr + 2 * sizeof(uint32_t) * nt;
+    HIPCHK(hipMemcpyAsync(b->d_rst, b->h_rst, bytes, hipMemcpyHostToDevice, b->stream));
+    DKCHK(dk_batch_restart((const dk_brestart *)b->d_rst,
+                           (const uint32_t *)(b->d_rst + sizeof(dk_brestart) * (size_t)nr), nt,
+                           b->stream));
+    b->nrst = 0;
+    b->rst_tiles = 0;
     return DOCI_OK;
 }
 
@@ -1058,7 +1125,7 @@ static int drain(doci_batch *b)
             finish(b, r, st);
         }
     }
-    return DOCI_OK;
+    return flush_restarts(b);
 }
 
 /* Pipelined: first finish the iteration queued by the previous call, then

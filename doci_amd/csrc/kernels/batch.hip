@@ -39,6 +39,7 @@
 namespace {
 
 constexpr int kBlock = 256;
+static_assert(kBlock == DK_ADMIT_ROWS, "one admission / restart row per thread");
 constexpr int kWaves = kBlock / 64;
 constexpr int kSmall = 64;
 constexpr int LDY = DK_BMAX_VEC;
@@ -711,24 +712,6 @@ __device__ void start_state(dk_bstate *st, const double *__restrict__ D, const i
     st->sio.active = 0;
 }
 
-// The first correction t (the guess: x, or the unit vector at argmin D) and,
-// from thread 0 of block 0, the start state: one launch per admitted instance.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_bguess(T *__restrict__ t, const double *__restrict__ x,
-                                                   uint32_t n, const int64_t *__restrict__ idx,
-                                                   dk_bstate *st, const double *__restrict__ D,
-                                                   int shift_min_d, double tol, double otol,
-                                                   int max_its, int keep, int max_vec, int its0)
-{
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i == 0) start_state(st, D, idx, shift_min_d, tol, otol, max_its, keep, max_vec, its0);
-    if (i >= n) return;
-    if (x)
-        t[i] = (T)x[i];
-    else
-        t[i] = (T)((int64_t)i == *idx ? 1.0 : 0.0);
-}
-
 // ---- admission of many instances: two launches --------------------------
 // First index of the minimum: (value, index) pairs, ties to the lower index.
 __device__ __forceinline__ void amin2(double &v, double &ix, double v2, double ix2)
@@ -826,6 +809,20 @@ __global__ __launch_bounds__(kBlock) void k_badmit_start(const dk_badmit *__rest
     }
 }
 
+// FP64 restart of mixed-precision instances: t = x over the tile's rows;
+// thread 0 of an instance's tile 0 writes its start state.
+__global__ __launch_bounds__(kBlock) void k_brestart(const dk_brestart *__restrict__ rs,
+                                                     const uint32_t *__restrict__ tiles)
+{
+    const dk_brestart &R = rs[tiles[2 * blockIdx.x]];
+    const uint32_t k = tiles[2 * blockIdx.x + 1];
+    if (k == 0 && threadIdx.x == 0)
+        start_state(R.state, nullptr, nullptr, 0, R.tol, R.otol, R.max_its, R.keep, R.max_vec,
+                    (int)R.its0);
+    const uint32_t I = k * DK_ADMIT_ROWS + threadIdx.x;
+    if (I < R.n) R.t[I] = R.x[I];
+}
+
 template <typename T, int M>
 int stream_launch(int op, const dk_binst *inst, dk_bstate *state, const uint32_t *tiles,
                   uint32_t ntile, double *part, hipStream_t s)
@@ -897,21 +894,11 @@ extern "C" int dk_batch_sigma_lane(int prec, uint32_t maxL, uint32_t maxN, const
     return (int)hipGetLastError();
 }
 
-extern "C" int dk_batch_start(const dk_binst *in, dk_bstate *state_i, int from_x, int prec,
-                              int shift_min_d, double tol, double otol, int max_its, int keep,
-                              int max_vec, int its0, double *red, int64_t *idx, hipStream_t s)
+extern "C" int dk_batch_restart(const dk_brestart *rs, const uint32_t *tiles, uint32_t ntile,
+                                hipStream_t s)
 {
-    int e = dk_argmin(in->D, (int64_t)in->n, red, idx, s);
-    if (e) return e;
-    const dim3 g((in->n + kBlock - 1) / kBlock), b(kBlock);
-    void *t = in->slots; // slot 0 holds t at admission
-    if (prec)
-        hipLaunchKernelGGL(k_bguess<float>, g, b, 0, s, (float *)t, from_x ? in->x : nullptr, in->n,
-                           idx, state_i, in->D, shift_min_d, tol, otol, max_its, keep, max_vec, its0);
-    else
-        hipLaunchKernelGGL(k_bguess<double>, g, b, 0, s, (double *)t, from_x ? in->x : nullptr,
-                           in->n, idx, state_i, in->D, shift_min_d, tol, otol, max_its, keep,
-                           max_vec, its0);
+    if (ntile == 0) return 0;
+    hipLaunchKernelGGL(k_brestart, dim3(ntile), dim3(kBlock), 0, s, rs, tiles);
     return (int)hipGetLastError();
 }
 

@@ -518,11 +518,24 @@ int orc_sygv2(const double *A, const double *B, double *w, double *Z)
 /* Davidson (src/davidson.c:33-255)                                          */
 /* ------------------------------------------------------------------------ */
 
+/* Sums over n entries in ORC_CHUNKS fixed chunks added in chunk order: the
+ * result does not depend on the thread count or on the order in which
+ * threads finish (an OpenMP reduction clause combines in arrival order,
+ * which makes rounding-sensitive runs differ from one call to the next). */
+#define ORC_CHUNKS 256
+#define CHUNK_LO(n, c) ((uint64_t)(n) * (uint64_t)(c) / ORC_CHUNKS)
+
 static double dot(const double *a, const double *b, uint64_t n)
 {
+    double part[ORC_CHUNKS];
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < ORC_CHUNKS; ++c) {
+        double s = 0;
+        for (uint64_t i = CHUNK_LO(n, c); i < CHUNK_LO(n, c + 1); ++i) s += a[i] * b[i];
+        part[c] = s;
+    }
     double s = 0;
-#pragma omp parallel for reduction(+ : s) schedule(static)
-    for (int64_t i = 0; i < (int64_t)n; ++i) s += a[i] * b[i];
+    for (int c = 0; c < ORC_CHUNKS; ++c) s += part[c];
     return s;
 }
 
@@ -602,8 +615,7 @@ int orc_davidson_timed(double *result, double *energy, int max_vec, int keep,
         }
         /* calculate_residue, davidson.c:195-212 */
         const double th = w[0];
-        double nr2 = 0;
-#pragma omp parallel for reduction(+ : nr2) schedule(static)
+#pragma omp parallel for schedule(static)
         for (int64_t k = 0; k < (int64_t)n; ++k) {
             double x = 0, r = 0;
             for (int j = 0; j < m; ++j) x += V[(size_t)k + (size_t)j * n] * Y[j];
@@ -611,9 +623,8 @@ int orc_davidson_timed(double *result, double *energy, int max_vec, int keep,
             r -= th * x;
             result[k] = x;
             t[k] = r;
-            nr2 += r * r;
         }
-        res = sqrt(nr2);
+        res = sqrt(dot(t, t, n));
         *energy = th;
         if (trace_res && its - 1 < trace_cap) {
             trace_res[its - 1] = res;

@@ -223,3 +223,26 @@ def test_batch_alone_equals_in_mix_star_instance():
         assert abs(mixr.energy - alone.energy) <= 1e-11, (prec, mixr.energy - alone.energy)
         assert np.abs(xa - xm).max() <= 1e-9, prec
         assert abs(abs(xa @ xm) - 1) <= 1e-12, prec
+
+
+def test_batch_engine_reused_across_batches(singles):
+    """One engine runs three batches back to back (FP64, mixed, FP64), as a
+    long-lived service does: the arena and the pinned admission / restart
+    lists are reused, and every batch gives the results of a fresh engine."""
+    print("[case] batch engine reuse", flush=True)
+    with da.Batch(tile_rows=1024) as fresh:
+        ids = [fresh.add_synthetic(L, N, s, precision="mixed") for L, N, s in CASES]
+        fresh.run()
+        ref_mixed = [fresh.result(i)[0].energy for i in ids]
+    with da.Batch(tile_rows=1024) as b:
+        for prec in ("fp64", "mixed", "fp64"):
+            ids = [b.add_synthetic(L, N, s, precision=prec) for L, N, s in CASES]
+            b.run()
+            for k, (i, c) in enumerate(zip(ids, CASES)):
+                r, x = b.result(i, want_x=True)
+                assert r.status == 2 and r.residual <= 1e-8, (prec, c)
+                assert abs(r.energy - singles[c].energy) <= 1e-9, (prec, c)
+                assert abs(abs(x @ singles[c].x) - 1) <= 1e-8, (prec, c)
+                if prec == "mixed":
+                    assert r.iterations_fp32 > 0
+                    assert abs(r.energy - ref_mixed[k]) <= 1e-11, c
