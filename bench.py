@@ -234,6 +234,10 @@ def main() -> int:
     log(f"operator L={L} N={N} seed={SEED}")
     t0 = time.perf_counter()
     op = da.Operator.synthetic(L, N, SEED)
+    # one untimed sigma per storage precision: the per-process loading of the
+    # kernels' code objects is set-up, not part of a solve (also with --warmup 0)
+    op.time_sigma(1, 1)
+    op.time_sigma(0, 1)
     log(f"setup {time.perf_counter() - t0:.2f} s, n={op.n}")
 
     def solve(prec):

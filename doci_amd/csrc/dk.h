@@ -271,10 +271,14 @@ int dk_ortho(int m, void *const *X, const double *g, void *y, int64_t n,
  * (H - shift) V, so with the Ritz value theta:
  *   x = sum_j a_j V_j, r = sum_j a_j W_j - (theta - shift) x,
  *   t = r / max-floor(|D - theta|, 1e-12)  written to `t`,
- *   out[0] = <r,r>, out[1] = <t,t>, out[2+j] = <V_j, t> (j < m).
- * When x_out is not NULL the Ritz vector x is also written there. */
+ *   out[0] = <r,r>, out[1] = <t,t>, out[2+j] = <V_j, t> (j < m), and when
+ *   wdots (m <= DK_RESIDUAL_WDOTS_MAX) also out[2+m+j] = <W_j, t>.
+ * Dots with t use t as stored (in the storage precision).  When x_out is not
+ * NULL the Ritz vector x is also written there.  red / out_dev hold
+ * 2 + 2m values. */
+#define DK_RESIDUAL_WDOTS_MAX 32
 int dk_residual(int m, void *const *V, void *const *W, const double *a,
-                double theta, double shift, const double *D, void *t, void *x_out,
+                double theta, double shift, const double *D, void *t, void *x_out, int wdots,
                 int64_t n, int prec, double *red, double *out_dev, hipStream_t s);
 
 /* out = sum_j g_j X_j (out may not alias any X_j); out has precision

@@ -159,9 +159,15 @@ static int dav_phase(dav_run *r, int prec, double shift, double stop_tol, double
     double *S = calloc((size_t)mv * mv, sizeof(double)), *G = calloc((size_t)mv * mv, sizeof(double));
     double *Y = calloc((size_t)mv * mv, sizeof(double)), *th = calloc((size_t)mv, sizeof(double));
     double *s = calloc((size_t)mv + 1, sizeof(double)), *a = calloc((size_t)mv + 2, sizeof(double));
-    double *red = calloc((size_t)mv + 8, sizeof(double)), *g = calloc((size_t)mv + 2, sizeof(double));
+    double *red = calloc(2 * (size_t)mv + 8, sizeof(double)), *g = calloc((size_t)mv + 2, sizeof(double));
     void **Vp = calloc((size_t)mv + 1, sizeof(void *)), **Wp = calloc((size_t)mv + 1, sizeof(void *));
-    if (!S || !G || !Y || !th || !s || !a || !red || !g || !Vp || !Wp) {
+    /* P: the projected matrix of the shifted operator in the stored vectors,
+     * P[i + j ld] = <V_i, W_j> (i <= j, unscaled); gam: the coefficients the
+     * orthogonalisation subtracted from t; bw: <W_j, t> from the residual */
+    double *P = calloc((size_t)mv * mv, sizeof(double)), *gam = calloc((size_t)mv + 2, sizeof(double));
+    double *bw = calloc((size_t)mv + 2, sizeof(double));
+    int have_b = 0;
+    if (!S || !G || !Y || !th || !s || !a || !red || !g || !Vp || !Wp || !P || !gam || !bw) {
         rc = doci_fail(DOCI_E_NOMEM, "Davidson host arrays");
         goto done;
     }
@@ -191,9 +197,15 @@ static int dav_phase(dav_run *r, int prec, double shift, double stop_tol, double
     const int start_its = *r->its;
     while (res > stop_tol && (*r->its)++ < o->max_its) {
         /* orthogonalise t against V (davidson.c:133-176) */
-        for (int j = 0; j < m; ++j) Vp[j] = w.Vs[w.basis[j]];
+        for (int j = 0; j < m; ++j) {
+            Vp[j] = w.Vs[w.basis[j]];
+            gam[j] = 0;
+        }
         for (int pass = 0; m > 0 && pass < 3; ++pass) {
-            for (int j = 0; j < m; ++j) g[j] = s[j] * s[j] * a[j];
+            for (int j = 0; j < m; ++j) {
+                g[j] = s[j] * s[j] * a[j];
+                gam[j] += g[j];
+            }
             DK(dk_ortho(m, Vp, g, w.Vs[tslot], n, prec, c->d_red, c->d_out, c->stream));
             CK(dh_fetch(c, m + 1, red));
             tt = red[0];
@@ -217,12 +229,31 @@ static int dav_phase(dav_run *r, int prec, double shift, double stop_tol, double
         Vp[m] = w.Vs[tslot];
         /* W_m = (H - shift) V_m */
         CK(timed_apply(r, w.Vs[tslot], w.Ws[tslot], prec, shift));
-        /* new column of V^T W (davidson.c:178-193) */
-        DK(dk_multidot(m + 1, Vp, w.Ws[tslot], n, prec, c->d_red, c->d_out, c->stream));
-        CK(dh_fetch(c, m + 2, red));
-        CK(add_sigma_time(r, prec));
+        /* new column of V^T W (davidson.c:178-193).  With the residual's
+         * <W_j, t> at hand, the entries j < m follow from the symmetry of H:
+         * <V_j, W_m> = <W_j, V_m> = <W_j, t> - sum_i gam_i <W_j, V_i>, and
+         * only <V_m, W_m> is streamed (two vectors instead of m + 2). */
+        if (have_b) {
+            void *Vm[1] = {Vp[m]};
+            DK(dk_multidot(1, Vm, w.Ws[tslot], n, prec, c->d_red, c->d_out, c->stream));
+            CK(dh_fetch(c, 2, red));
+            CK(add_sigma_time(r, prec));
+            for (int j = 0; j < m; ++j) {
+                double v = bw[j];
+                for (int i = 0; i < m; ++i)
+                    v -= gam[i] * (i <= j ? P[i + (size_t)j * ld] : P[j + (size_t)i * ld]);
+                P[j + (size_t)m * ld] = v;
+            }
+            P[m + (size_t)m * ld] = red[1];
+        } else {
+            DK(dk_multidot(m + 1, Vp, w.Ws[tslot], n, prec, c->d_red, c->d_out, c->stream));
+            CK(dh_fetch(c, m + 2, red));
+            CK(add_sigma_time(r, prec));
+            for (int i = 0; i <= m; ++i) P[i + (size_t)m * ld] = red[1 + i];
+        }
+        have_b = 0;
         for (int i = 0; i <= m; ++i)
-            S[i + (size_t)m * ld] = s[i] * s[m] * red[1 + i] + shift * G[i + (size_t)m * ld];
+            S[i + (size_t)m * ld] = s[i] * s[m] * P[i + (size_t)m * ld] + shift * G[i + (size_t)m * ld];
         ++m;
         CK(dh_gen_eig(m, S, G, ld, th, Y, ld));
         if (m == mv) {
@@ -252,8 +283,10 @@ static int dav_phase(dav_run *r, int prec, double shift, double stop_tol, double
                 for (int i = 0; i < k; ++i) G[i + (size_t)k * ld] = red[1 + i];
                 DK(dk_multidot(k + 1, Vp, w.Ws[nb[k]], n, prec, c->d_red, c->d_out, c->stream));
                 CK(dh_fetch(c, k + 2, red));
-                for (int i = 0; i <= k; ++i)
+                for (int i = 0; i <= k; ++i) {
+                    P[i + (size_t)k * ld] = red[1 + i];
                     S[i + (size_t)k * ld] = red[1 + i] + shift * G[i + (size_t)k * ld];
+                }
             }
             CK(dh_gen_eig(m, S, G, ld, th, Y, ld));
         }
@@ -275,12 +308,19 @@ static int dav_phase(dav_run *r, int prec, double shift, double stop_tol, double
             tt = red[0];
             for (int j = 0; j < m; ++j) a[j] = red[1 + j];
         } else {
-            DK(dk_residual(m, Vp, Wp, g, theta, shift, c->d_D, w.Vs[tslot], NULL, n, prec, c->d_red,
-                           c->d_out, c->stream));
-            CK(dh_fetch(c, m + 2, red));
+            /* FP64 storage only: with FP32 storage the two ways of forming the
+             * column differ by the rounding of the stored W (~1e-7 relative),
+             * which moves the FP32 phase's iterates; its explicit dots are
+             * half the bytes anyway */
+            const int wd = prec == 0 && m <= DK_RESIDUAL_WDOTS_MAX;
+            DK(dk_residual(m, Vp, Wp, g, theta, shift, c->d_D, w.Vs[tslot], NULL, wd, n, prec,
+                           c->d_red, c->d_out, c->stream));
+            CK(dh_fetch(c, wd ? 2 + 2 * m : m + 2, red));
             res = sqrt(red[0]);
             tt = red[1];
             for (int j = 0; j < m; ++j) a[j] = red[2 + j];
+            for (int j = 0; wd && j < m; ++j) bw[j] = red[2 + m + j];
+            have_b = wd;
         }
         r->info->d_energy = *energy - theta; /* davidson.c:92-93 */
         *energy = theta;
@@ -314,6 +354,9 @@ done:
     free(g);
     free(Vp);
     free(Wp);
+    free(P);
+    free(gam);
+    free(bw);
     return rc;
 }
 

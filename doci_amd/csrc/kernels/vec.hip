@@ -14,6 +14,9 @@ namespace {
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
 constexpr int kMaxBlocks = 2048;
+// the residual's 2 + 2m values fit the reduction buffers the host sizes for
+// DK_MAX_VEC + 8 values
+static_assert(2 + 2 * DK_RESIDUAL_WDOTS_MAX <= DK_MAX_VEC + 8, "residual dots");
 
 struct Ptrs {
     const void *p[DK_MAX_VEC];
@@ -57,6 +60,35 @@ __device__ __forceinline__ void block_partials(double (&acc)[NV], int nv, double
         double s = 0;
 #pragma unroll
         for (int w = 0; w < kWaves; ++w) s += sh[k][w];
+        red[(size_t)blockIdx.x * nv + k] = s;
+    }
+}
+
+// The same for the residual with W dots: acc holds <r,r>, <t,t>, M slots of
+// V dots and M of W dots; the partials are stored packed, 2 + 2m per block
+// (the W dots right after the m V dots).
+template <int M>
+__device__ __forceinline__ void block_partials_vw(double (&acc)[2 + 2 * M], int m, double *red)
+{
+    __shared__ double sh[2 + 2 * M][kWaves];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 2 + 2 * M; ++k) {
+        const int j = k < 2 + M ? k - 2 : k - 2 - M;
+        if (j < m) {
+            double v = acc[k];
+#pragma unroll
+            for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+            if (lane == 0) sh[k][wave] = v;
+        }
+    }
+    __syncthreads();
+    const int nv = 2 + 2 * m;
+    for (int k = threadIdx.x; k < nv; k += blockDim.x) {
+        const int kin = k < 2 + m ? k : k + (M - m);
+        double s = 0;
+#pragma unroll
+        for (int w = 0; w < kWaves; ++w) s += sh[kin][w];
         red[(size_t)blockIdx.x * nv + k] = s;
     }
 }
@@ -200,12 +232,24 @@ __global__ __launch_bounds__(kBlock) void k_ortho(Ptrs X, int m, Coef g, T *__re
 }
 
 // ---- Davidson residual (src/davidson.c:195-227 fused) ---------------------
-// out[0] = <r,r>, out[1] = <t,t>, out[2+j] = <V_j,t>
-template <typename T, int M, int VW>
+// out[0] = <r,r>, out[1] = <t,t>, out[2+j] = <V_j,t>; with WB also
+// <W_j,t> (acc[2+M+j]), from W_j loaded a second time once t is known: the
+// lines are still in L2, and holding the W values in registers instead
+// costs more occupancy than the second loads cost (measured at L = 34: m = 8
+// residual 4.26 ms with the second loads, 4.60 ms with the values held).
+template <typename P>
+__device__ __forceinline__ P launder(P p)
+{
+    asm volatile("" : "+v"(p));
+    return p;
+}
+
+template <typename T, int M, int VW, bool WB>
 __device__ __forceinline__ void residual_step(const Ptrs &V, const Ptrs &W, int m, const Coef &a,
                                               double theta, double shift,
                                               const double *__restrict__ D, T *__restrict__ t,
-                                              T *__restrict__ xo, int64_t q, double (&acc)[M + 2])
+                                              T *__restrict__ xo, int64_t q,
+                                              double (&acc)[2 + (WB ? 2 : 1) * M])
 {
     typedef typename Vec<T, VW>::type VT;
     VT vv[M];
@@ -243,25 +287,36 @@ __device__ __forceinline__ void residual_step(const Ptrs &V, const Ptrs &W, int 
         if (j < m)
 #pragma unroll
             for (int e = 0; e < VW; ++e) acc[2 + j] += el<T, VW>(vv[j], e) * td[e];
+    if constexpr (WB) {
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+            if (j < m) {
+                const VT wv = ldv<T, VW>(launder(W.p[j]), q);
+#pragma unroll
+                for (int e = 0; e < VW; ++e) acc[2 + M + j] += el<T, VW>(wv, e) * td[e];
+            }
+    }
 }
 
-template <typename T, int M>
+template <typename T, int M, bool WB>
 __global__ __launch_bounds__(kBlock) void k_residual(Ptrs V, Ptrs W, int m, Coef a, double theta,
                                                      double shift, const double *__restrict__ D,
                                                      T *__restrict__ t, T *__restrict__ xo,
                                                      int64_t n, double *__restrict__ red)
 {
     constexpr int VW = vw_for<T, M>();
-    double acc[M + 2];
+    constexpr int NA = 2 + (WB ? 2 : 1) * M;
+    double acc[NA];
 #pragma unroll
-    for (int k = 0; k < M + 2; ++k) acc[k] = 0;
+    for (int k = 0; k < NA; ++k) acc[k] = 0;
     const int64_t stride = (int64_t)gridDim.x * kBlock, nq = n / VW;
     const int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
     for (int64_t q = tid; q < nq; q += stride)
-        residual_step<T, M, VW>(V, W, m, a, theta, shift, D, t, xo, q, acc);
+        residual_step<T, M, VW, WB>(V, W, m, a, theta, shift, D, t, xo, q, acc);
     for (int64_t i = nq * VW + tid; i < n; i += stride)
-        residual_step<T, M, 1>(V, W, m, a, theta, shift, D, t, xo, i, acc);
-    block_partials<M + 2>(acc, m + 2, red);
+        residual_step<T, M, 1, WB>(V, W, m, a, theta, shift, D, t, xo, i, acc);
+    if constexpr (WB) block_partials_vw<M>(acc, m, red);
+    else block_partials<M + 2>(acc, m + 2, red);
 }
 
 template <typename T, typename To, int M, int VW>
@@ -512,21 +567,34 @@ extern "C" int dk_ortho(int m, void *const *X, const double *g, void *y, int64_t
 }
 
 extern "C" int dk_residual(int m, void *const *V, void *const *W, const double *a, double theta,
-                           double shift, const double *D, void *t, void *x_out, int64_t n, int prec,
-                           double *red, double *out, hipStream_t s)
+                           double shift, const double *D, void *t, void *x_out, int wdots, int64_t n,
+                           int prec, double *red, double *out, hipStream_t s)
 {
-    if (m < 1 || m > DK_MAX_VEC) return (int)hipErrorInvalidValue;
+    if (m < 1 || m > DK_MAX_VEC || (wdots && m > DK_RESIDUAL_WDOTS_MAX))
+        return (int)hipErrorInvalidValue;
     const int nb = grid_for(n);
     const Ptrs PV = make_ptrs(m, V), PW = make_ptrs(m, W);
     const Coef A = make_coef(m, a);
-    if (prec == 0)
-        DK_BUCKET(m, hipLaunchKernelGGL((k_residual<double, MM>), dim3(nb), dim3(kBlock), 0, s, PV, PW,
-                                        m, A, theta, shift, D, (double *)t, (double *)x_out, n, red));
-    else
-        DK_BUCKET(m, hipLaunchKernelGGL((k_residual<float, MM>), dim3(nb), dim3(kBlock), 0, s, PV, PW,
-                                        m, A, theta, shift, D, (float *)t, (float *)x_out, n, red));
+#define DK_RES_LAUNCH(T, WB)                                                                  \
+    hipLaunchKernelGGL((k_residual<T, MM, WB>), dim3(nb), dim3(kBlock), 0, s, PV, PW, m, A, theta, \
+                       shift, D, (T *)t, (T *)x_out, n, red)
+#define DK_BUCKET32(m, CALL)                                  \
+    do {                                                      \
+        if ((m) <= 8) { constexpr int MM = 8; CALL; }         \
+        else if ((m) <= 16) { constexpr int MM = 16; CALL; }  \
+        else { constexpr int MM = 32; CALL; }                 \
+    } while (0)
+    if (wdots) {
+        if (prec == 0) DK_BUCKET32(m, DK_RES_LAUNCH(double, true));
+        else DK_BUCKET32(m, DK_RES_LAUNCH(float, true));
+    } else {
+        if (prec == 0) DK_BUCKET(m, DK_RES_LAUNCH(double, false));
+        else DK_BUCKET(m, DK_RES_LAUNCH(float, false));
+    }
+#undef DK_BUCKET32
+#undef DK_RES_LAUNCH
     int e = (int)hipGetLastError();
-    return e ? e : finalize(red, nb, m + 2, out, s);
+    return e ? e : finalize(red, nb, wdots ? 2 + 2 * m : m + 2, out, s);
 }
 
 extern "C" int dk_lincomb(int m, void *const *X, const double *g, void *out, int64_t n, int prec,
