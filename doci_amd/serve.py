@@ -9,10 +9,11 @@ the device-memory budget (FIFO), runs it in lock step with everything else
 that is running, and retires it when it converges.  Time-to-converge of a
 request = its completion time - its arrival time, on the batch's clock.
 
-`one_at_a_time` and `one_at_a_time_queue` time the baseline the stream is
-compared with: the same requests solved one after the other as plain FP64
-single-instance solves (the second as a real-time FIFO server fed by the
-same arrivals).
+`one_at_a_time_queue` times the baseline the stream is compared with: the
+same requests solved one after the other as plain FP64 single-instance
+solves, as a real-time FIFO server fed by given arrivals (all at 0: back to
+back); `fifo_latencies` replays measured service times against other
+arrivals.
 """
 from __future__ import annotations
 
@@ -20,7 +21,7 @@ import time
 
 import numpy as np
 
-from . import Batch, Operator, gen_integrals, read_fcidump
+from . import Batch, Operator, binom, gen_integrals, read_fcidump
 
 
 def _integrals(req):
@@ -90,20 +91,6 @@ def run_stream(requests, arrivals, precision="fp64", integrals=None, **batch_opt
                 steps=steps, tile_rows=st["tile_rows"])
 
 
-def one_at_a_time(requests) -> dict:
-    """Plain FP64 single-instance solves one after the other (operator
-    build from in-memory integrals + Davidson), each timed."""
-    times, energies = [], []
-    for req in requests:
-        L, N, core, one, two = _integrals(req)
-        t = time.perf_counter()
-        with Operator.from_integrals(L, N, core, one, two) as op:
-            r = op.solve("D")
-        times.append(time.perf_counter() - t)
-        energies.append(r.energy)
-    return dict(service=np.array(times), energy=np.array(energies))
-
-
 def one_at_a_time_queue(requests, arrivals, integrals=None) -> dict:
     """A one-at-a-time server measured in real time: requests are solved in
     arrival order as plain FP64 single-instance solves (operator build from
@@ -143,24 +130,50 @@ def fifo_latencies(arrivals, service) -> np.ndarray:
 def main(argv=None) -> int:
     """python -m doci_amd.serve [--requests R] [--rate X] [--precision P]:
     the serving workload's first R requests (doci_amd.workloads) at Poisson
-    rate X times the one-at-a-time FP64 throughput of a sample; prints
-    solves/s and p50/p99 time-to-converge."""
+    rate X times mu0, the one-at-a-time plain FP64 throughput of the same
+    requests solved back to back; prints solves/s and p50/p99
+    time-to-converge beside those of a one-at-a-time FIFO server with the
+    measured service times and the same arrivals."""
     import argparse
     from .workloads import poisson_arrivals, serving_requests
     ap = argparse.ArgumentParser(prog="python -m doci_amd.serve")
     ap.add_argument("--requests", type=int, default=500)
     ap.add_argument("--rate", type=float, default=10.0)
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--by-size", action="store_true",
+                    help="also serve the requests from 2^20 determinants on and the smaller "
+                         "ones as two separate sets, all arriving at once, and print each "
+                         "set's engine time beside its one-at-a-time service time")
     a = ap.parse_args(argv)
     reqs = serving_requests(a.requests)
-    sample = reqs[:: max(1, a.requests // 40)]
-    one_at_a_time(sample)  # warm the per-shape tables
-    mu0 = 1.0 / float(one_at_a_time(sample)["service"].mean())
-    res = run_stream(reqs, poisson_arrivals(a.requests, a.rate * mu0), a.precision)
+    ints = [_integrals(q) for q in reqs]
+    first = {}
+    for k, q in enumerate(reqs):
+        first.setdefault(q[:2], k)
+    one_at_a_time_queue([reqs[k] for k in first.values()], [0.0] * len(first),
+                        integrals=[ints[k] for k in first.values()])  # per-shape tables
+    service = one_at_a_time_queue(reqs, [0.0] * len(reqs), integrals=ints)["service"]
+    mu0 = len(reqs) / float(service.sum())
+    arr = poisson_arrivals(a.requests, a.rate * mu0)
+    res = run_stream(reqs, arr, a.precision, integrals=list(ints))
+    one = fifo_latencies(arr, service)
     print(f"{a.requests} requests at {a.rate} x {mu0:.1f}/s ({a.precision}): "
           f"{res['solves_per_s']:.1f} solves/s, p50 {res['p50']:.3f} s, p99 {res['p99']:.3f} s, "
           f"peak running {res['peak_running']}, all converged "
-          f"{bool((res['status'] == 2).all())}", flush=True)
+          f"{bool((res['status'] == 2).all())}; one at a time: "
+          f"{a.requests / float(np.max(np.asarray(arr) + one)):.1f} solves/s, "
+          f"p50 {np.percentile(one, 50):.3f} s, p99 {np.percentile(one, 99):.3f} s", flush=True)
+    if a.by_size:
+        n = np.array([binom(q[1], q[0]) for q in ints])
+        for name, sel in (("n >= 2^20", n >= (1 << 20)), ("n < 2^20", n < (1 << 20))):
+            idx = np.nonzero(sel)[0]
+            if not len(idx):
+                continue
+            r = run_stream([reqs[i] for i in idx], [0.0] * len(idx), a.precision,
+                           integrals=[ints[i] for i in idx])
+            print(f"{name}: {len(idx)} requests, one at a time {float(service[idx].sum()):.3f} s "
+                  f"of service, engine ({a.precision}, all at once) "
+                  f"{len(idx) / r['solves_per_s']:.3f} s", flush=True)
     return 0
 
 
