@@ -417,6 +417,19 @@ class Batch:
         core, one, two = gen_integrals(L, seed)
         return self.add(L, N, core, one, two, precision)
 
+    def add_pipelined(self, items, precision: str = "fp64", chunk: int = 64) -> list:
+        """Queues many instances ((L, N, core, one, two) tuples) in chunks,
+        stepping the engine after each chunk but the last, so that the host
+        intake of a chunk (the integral transform) overlaps the GPU's
+        iterations of the chunks before it.  Returns the instance ids."""
+        items = list(items)
+        ids = []
+        for c0 in range(0, len(items), chunk):
+            ids += [self.add(*it, precision=precision) for it in items[c0:c0 + chunk]]
+            if c0 + chunk < len(items):
+                self.step()
+        return ids
+
     def step(self):
         r, w = C.c_int(), C.c_int()
         _check(lib().doci_batch_step(self._h, C.byref(r), C.byref(w)), "doci_batch_step")

@@ -246,3 +246,18 @@ def test_batch_engine_reused_across_batches(singles):
                 if prec == "mixed":
                     assert r.iterations_fp32 > 0
                     assert abs(r.energy - ref_mixed[k]) <= 1e-11, c
+
+
+def test_batch_add_pipelined_matches_all_at_once(singles):
+    """Batch.add_pipelined: instances queued in chunks with engine steps in
+    between give every instance the result of its single solve."""
+    print("[case] batch add_pipelined", flush=True)
+    items = [(L, N) + tuple(da.gen_integrals(L, s)) for L, N, s in CASES]
+    with da.Batch(tile_rows=1024) as b:
+        ids = b.add_pipelined(items, chunk=3)
+        b.run()
+        for i, c in zip(ids, CASES):
+            r, x = b.result(i, want_x=True)
+            assert r.status == 2 and r.residual <= 1e-8, c
+            assert abs(r.energy - singles[c].energy) <= 1e-9, c
+            assert abs(abs(x @ singles[c].x) - 1) <= 1e-8, c
