@@ -82,12 +82,17 @@ typedef struct {
     uint32_t a, b;
     uint32_t ngroup;            /* main-pass groups */
     uint32_t emax;              /* determinants staged per main-pass group */
+    uint32_t gslabs;            /* slabs per main-pass group, at most */
+    uint32_t owns;              /* LDS doubles of a main-pass group's staged slabs (padded layout) */
     uint32_t nsgroup;           /* star-pass groups */
     uint32_t sgmax;             /* stars per star group, at most */
     uint32_t selems;            /* LDS elements staged per star group, at most */
     uint32_t ymax;              /* member-table entries per star (= a) */
     uint32_t pass;              /* star pass: 0 column pass (wave-uniform K), 1 per-star vector,
-                                   2 per-star FP64 matrix cores, 3 FP32 column pass on the matrix cores */
+                                   2 per-star FP64 matrix cores, 3 FP32 column pass on the matrix cores,
+                                   4 column pass with 64-bit addresses (0 uses 32-bit buffer offsets
+                                   where the vector and every member sector of Z fit them) */
+    uint32_t zsec_max;          /* largest member sector of Z, zoff[k+1] - zoff[k] (elements) */
     const uint32_t *slab_start; /* per slab (order: k, then lexicographic alpha) */
     const uint32_t *slab_zbase;
     const uint32_t *group;      /* main groups: first slab, nslab | m << 16 */

@@ -19,7 +19,8 @@
 #define STAR_BMAX 16
 #define STAR_PASS_DEFAULT 0 /* column star pass (tuning key "star_pass") */
 #define STAR_SGMAX_DEFAULT 64
-#define MAIN_EMAX_DEFAULT 1024  /* raised to the biggest slab */
+#define MAIN_EMAX_DEFAULT 4096  /* raised to the biggest slab */
+#define MAIN_GSLABS 256         /* slabs per main-pass group, at most */
 
 /* Device tables of one shape (L, N, limits).  They do not depend on the
  * integrals, so every operator of that shape shares them; shapes are built
@@ -55,6 +56,16 @@ static int cmp_sgrp(const void *x, const void *y)
     return a->start < b->start ? -1 : a->start > b->start;
 }
 
+/* LDS doubles a main-pass group of nsl slabs of S stages: sets of 4 (2, 1)
+ * slabs side by side at a stride of 5 (3, 1) doubles (main_spl, main_stride
+ * of the kernels) */
+static uint64_t own_doubles(uint64_t nsl, uint64_t S)
+{
+    const uint64_t spl = (nsl & 3) == 0 ? 4 : (nsl & 1) == 0 ? 2 : 1;
+    const uint64_t str = spl == 1 ? 1 : spl + 1;
+    return nsl / spl * S * str;
+}
+
 static int valid_sector(unsigned k, unsigned N, unsigned a, unsigned b)
 {
     return k <= N && k <= a && N - k <= b;
@@ -65,8 +76,10 @@ static unsigned nslot(unsigned k, unsigned m, unsigned a) { return k + ((m >= 1 
 static size_t star_lds_bytes(unsigned L, unsigned N, unsigned sgmax, unsigned ymax, unsigned selems)
 {
 #define A8(x) (((size_t)(x) + 7) & ~(size_t)7)
-    return A8(8 * (size_t)L * L) + A8(4 * (size_t)(L + 1) * (N + 2)) + 2 * A8(8 * (size_t)selems) +
-           2 * A8(4 * (size_t)sgmax * ymax) + 2 * A8(4 * (size_t)sgmax) +
+    /* + 64 zero / scratch centre entries; member tables with one more row of
+     * absent members (the buffer-addressed column pass) */
+    return A8(8 * (size_t)L * L) + A8(4 * (size_t)(L + 1) * (N + 2)) + 2 * A8(8 * ((size_t)selems + 64)) +
+           2 * A8(4 * ((size_t)sgmax + 1) * ymax) + 2 * A8(4 * (size_t)sgmax) +
            A8((size_t)sgmax * (ymax > 8 ? ymax : 8));
 #undef A8
 }
@@ -212,8 +225,8 @@ static int build_shape(const dk_tables *dt, unsigned lds_budget, unsigned sgmax,
         if (maxS < MAIN_EMAX) maxS = MAIN_EMAX;
         if (ymax > 64) continue;
         if (star_lds_bytes(L, N, STAR_SGMAX, ymax, (unsigned)mx) <= STAR_LDS_BUDGET &&
-            8 * (size_t)b * b + 4 * (size_t)(b + 1) * (N + 2) + 16 * (size_t)maxS + 64 <=
-                MAIN_LDS_MAX) {
+            8 * (size_t)b * b + 4 * (size_t)(b + 1) * (N + 2) + 12 * (size_t)maxS +
+                    8 * (size_t)MAIN_GSLABS + 64 <= MAIN_LDS_MAX) {
             /* the rest of the budget stages as many centres as fit */
             selems = (unsigned)((STAR_LDS_BUDGET - star_lds_bytes(L, N, STAR_SGMAX, ymax, 0)) / 16);
             if (selems < mx) selems = (unsigned)mx;
@@ -260,12 +273,19 @@ static int build_shape(const dk_tables *dt, unsigned lds_budget, unsigned sgmax,
         rc = doci_fail(DOCI_E_NOMEM, "star tables");
         goto out;
     }
-    /* slabs and main-pass groups */
-    uint64_t is = 0, ng = 0;
+    /* slabs and main-pass groups: as many whole slabs of one sector as fit
+     * emax determinants, at most MAIN_GSLABS, rounded down to a multiple of 4
+     * (or to 2): the main pass walks a slab's hops once for a set of 4 (2)
+     * slabs of the group */
+    uint64_t is = 0, ng = 0, gslabs = 1, owns = 0;
     for (unsigned k = 0; k <= a; ++k) {
         if (!valid_sector(k, N, a, b)) continue;
         const unsigned m = N - k;
         const uint64_t S = dh_binom(m, b), ns = nslot(k, m, a);
+        uint64_t pg = emax / S;
+        if (pg > MAIN_GSLABS) pg = MAIN_GSLABS;
+        pg = pg >= 4 ? pg & ~(uint64_t)3 : pg >= 2 ? 2 : 1;
+        if (pg > gslabs) gslabs = pg;
         secbase[k] = is;
         for (unsigned e = 0; e < k; ++e) idx[e] = (int)e;
         uint32_t gfirst = (uint32_t)is, gcount = 0;
@@ -273,7 +293,9 @@ static int build_shape(const dk_tables *dt, unsigned lds_budget, unsigned sgmax,
         do {
             uint64_t sa = 0;
             for (unsigned e = 0; e < k; ++e) sa += dh_binom(N - e, L - 1 - (unsigned)idx[e]);
-            if (gcount == 0xffff || (gcount + 1) * S > emax) {
+            if (gcount == pg) {
+                const uint64_t ow = own_doubles(gcount, S);
+                if (ow > owns) owns = ow;
                 grp[ng].first = gfirst;
                 grp[ng].start = sstart[gfirst];
                 grp[ng].word = gcount | (m << 16);
@@ -287,6 +309,10 @@ static int build_shape(const dk_tables *dt, unsigned lds_budget, unsigned sgmax,
             ++li;
             ++gcount;
         } while (next_comb(idx, k, a));
+        {
+            const uint64_t ow = own_doubles(gcount, S);
+            if (ow > owns) owns = ow;
+        }
         grp[ng].first = gfirst;
         grp[ng].start = sstart[gfirst];
         grp[ng].word = gcount | (m << 16);
@@ -381,6 +407,8 @@ static int build_shape(const dk_tables *dt, unsigned lds_budget, unsigned sgmax,
         s->d.b = b;
         s->d.ngroup = (uint32_t)ng;
         s->d.emax = (uint32_t)emax;
+        s->d.gslabs = (uint32_t)gslabs;
+        s->d.owns = (uint32_t)owns;
         s->d.nsgroup = (uint32_t)nsg;
         s->d.sgmax = STAR_SGMAX;
         jgroup[N] = (uint32_t)nsg;
@@ -388,6 +416,9 @@ static int build_shape(const dk_tables *dt, unsigned lds_budget, unsigned sgmax,
         s->d.selems = selems;
         s->d.ymax = ymax;
         s->d.pass = STAR_PASS_DEFAULT;
+        s->d.zsec_max = 0;
+        for (unsigned k = 0; k <= a; ++k)
+            if (zoff[k + 1] - zoff[k] > s->d.zsec_max) s->d.zsec_max = zoff[k + 1] - zoff[k];
         s->d.star_amask = (const uint64_t *)d;
         s->d.slab_start = (const uint32_t *)(d + b_am);
         s->d.slab_zbase = (const uint32_t *)(d + b_am + b_ss);
@@ -528,7 +559,7 @@ int dh_stars_set_pass(doci_op *op, int pass)
         const int rc = dh_stars_build(op, 0, 0, 0);
         if (rc != DOCI_OK) return rc;
     }
-    if (pass < 0 || pass > 3) return doci_fail(DOCI_E_ARG, "star_pass must be 0 to 3");
+    if (pass < 0 || pass > 4) return doci_fail(DOCI_E_ARG, "star_pass must be 0 to 4");
     ((dh_stars *)op->stars)->pass = pass;
     return DOCI_OK;
 }

@@ -39,24 +39,31 @@ int launch(const dk_tables *t, const dk_stars *st, const double *D, double shift
         if (e) return e;
         j = N; // the other star passes below are skipped
     }
-    if ((st->pass == 0 || st->pass == 3) && j < N && a >= 1 && a <= kColAMax) {
+    if ((st->pass == 0 || st->pass == 3 || st->pass == 4) && j < N && a >= 1 && a <= kColAMax) {
         // one launch for every star group: K packed behind the slots, then the column pass
+        // (buffer-addressed where the vector and every members' Z sector fit 32-bit offsets)
         double *kp = (double *)((char *)Z + sizeof(double) * (size_t)st->zoff[a + 1]);
         const dim3 grid(st->nsgroup);
+        const int buf = st->pass != 4 && (uint64_t)t->n * sizeof(T) < kBufLimit &&
+                        (uint64_t)st->zsec_max * sizeof(T) < kBufLimit;
         int done = -1;
         if (col_exact_shape(a, b)) {
-            done = dk_col_exact_lo(sizeof(T) == 4, kChecked, a, b, grid, ls, s, *t, *st, c, Z, bad,
-                                   io, kp);
+            done = dk_col_exact_lo(sizeof(T) == 4, kChecked, buf, a, b, grid, ls, s, *t, *st, c, Z,
+                                   bad, io, kp);
             if (done < 0)
-                done = dk_col_exact_hi(sizeof(T) == 4, kChecked, a, b, grid, ls, s, *t, *st, c, Z,
-                                       bad, io, kp);
+                done = dk_col_exact_hi(sizeof(T) == 4, kChecked, buf, a, b, grid, ls, s, *t, *st, c,
+                                       Z, bad, io, kp);
         }
         const int am = done < 0 ? col_amax(a) : 0, bm = col_bmax(b);
 #define DK_COL_CASE(AM, BM)                                                                        \
     if (am == AM && bm == BM) {                                                                    \
         hipLaunchKernelGGL((k_kpack<AM, BM>), dim3(1), dim3(kBlock), 0, s, *t, a, b, kp);          \
-        hipLaunchKernelGGL((k_star_col<T, kChecked, AM, BM>), grid, dim3(kBlock), ls, s, *t, *st, \
-                           c, Z, bad, io, (const double *)kp);                                     \
+        if (buf)                                                                                   \
+            hipLaunchKernelGGL((k_star_colb<T, kChecked, AM, BM, false>), grid, dim3(kBlock), ls, s, *t,  \
+                               *st, c, Z, bad, io, (const double *)kp);                            \
+        else                                                                                       \
+            hipLaunchKernelGGL((k_star_col<T, kChecked, AM, BM>), grid, dim3(kBlock), ls, s, *t,   \
+                               *st, c, Z, bad, io, (const double *)kp);                            \
     }
         DK_COL_CASE(8, 12)
         DK_COL_CASE(8, 16)

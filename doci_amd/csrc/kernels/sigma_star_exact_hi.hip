@@ -8,11 +8,26 @@
 namespace {
 
 template <int AM, int BM>
-int launch_col(int prec, int checked, int a, int b, dim3 grid, size_t ls, hipStream_t s,
+int launch_col(int prec, int checked, int buf, int a, int b, dim3 grid, size_t ls, hipStream_t s,
                const dk_tables &t, const dk_stars &st, const void *c, void *Z, uint32_t *bad,
                const dk_sigma_io *io, double *kp)
 {
     hipLaunchKernelGGL((k_kpack<AM, BM>), dim3(1), dim3(kBlock), 0, s, t, a, b, kp);
+    if (buf) {
+        if (prec == 0 && !checked)
+            hipLaunchKernelGGL((k_star_colb<double, false, AM, BM, true>), grid, dim3(kBlock), ls, s, t, st,
+                               (const double *)c, (double *)Z, bad, io, (const double *)kp);
+        else if (prec == 0)
+            hipLaunchKernelGGL((k_star_colb<double, true, AM, BM, true>), grid, dim3(kBlock), ls, s, t, st,
+                               (const double *)c, (double *)Z, bad, io, (const double *)kp);
+        else if (!checked)
+            hipLaunchKernelGGL((k_star_colb<float, false, AM, BM, true>), grid, dim3(kBlock), ls, s, t, st,
+                               (const float *)c, (float *)Z, bad, io, (const double *)kp);
+        else
+            hipLaunchKernelGGL((k_star_colb<float, true, AM, BM, true>), grid, dim3(kBlock), ls, s, t, st,
+                               (const float *)c, (float *)Z, bad, io, (const double *)kp);
+        return 0;
+    }
     if (prec == 0 && !checked)
         hipLaunchKernelGGL((k_star_col<double, false, AM, BM>), grid, dim3(kBlock), ls, s, t, st,
                            (const double *)c, (double *)Z, bad, io, (const double *)kp);
@@ -30,17 +45,17 @@ int launch_col(int prec, int checked, int a, int b, dim3 grid, size_t ls, hipStr
 
 } // namespace
 
-int dk_col_exact_hi(int prec, int checked, int a, int b, dim3 grid, size_t lds, hipStream_t s,
+int dk_col_exact_hi(int prec, int checked, int buf, int a, int b, dim3 grid, size_t lds, hipStream_t s,
                     const dk_tables &t, const dk_stars &st, const void *c, void *Z, uint32_t *bad,
                     const dk_sigma_io *io, double *kp)
 {
     if (b != 13) return -1;
     switch (a) {
-    case 17: return launch_col<17, 13>(prec, checked, a, b, grid, lds, s, t, st, c, Z, bad, io, kp);
-    case 18: return launch_col<18, 13>(prec, checked, a, b, grid, lds, s, t, st, c, Z, bad, io, kp);
-    case 19: return launch_col<19, 13>(prec, checked, a, b, grid, lds, s, t, st, c, Z, bad, io, kp);
-    case 20: return launch_col<20, 13>(prec, checked, a, b, grid, lds, s, t, st, c, Z, bad, io, kp);
-    case 21: return launch_col<21, 13>(prec, checked, a, b, grid, lds, s, t, st, c, Z, bad, io, kp);
+    case 17: return launch_col<17, 13>(prec, checked, buf, a, b, grid, lds, s, t, st, c, Z, bad, io, kp);
+    case 18: return launch_col<18, 13>(prec, checked, buf, a, b, grid, lds, s, t, st, c, Z, bad, io, kp);
+    case 19: return launch_col<19, 13>(prec, checked, buf, a, b, grid, lds, s, t, st, c, Z, bad, io, kp);
+    case 20: return launch_col<20, 13>(prec, checked, buf, a, b, grid, lds, s, t, st, c, Z, bad, io, kp);
+    case 21: return launch_col<21, 13>(prec, checked, buf, a, b, grid, lds, s, t, st, c, Z, bad, io, kp);
     default: return -1;
     }
 }

@@ -29,6 +29,17 @@
 namespace {
 
 constexpr int kBlock = 256;
+constexpr int kCtrPad = 64;
+// Byte offset of an absent member (or a lane past the last column) in the
+// buffer-addressed column pass: beyond every buffer's range, so the
+// hardware range check returns 0 for a load and drops a store.  Buffers must
+// stay below kBufLimit bytes (checked by the launcher), and an offset within
+// a slab (< 2^16 elements of at most 8 bytes) added to kOob does not wrap.
+constexpr uint32_t kOob = 0xfff00000u;
+constexpr uint64_t kBufLimit = 0xffe00000ull;
+#ifndef DK_COLB_WAVES
+#define DK_COLB_WAVES 3
+#endif
 
 __host__ __device__ __forceinline__ size_t a8(size_t x) { return (x + 7) & ~(size_t)7; }
 
@@ -85,10 +96,13 @@ __host__ __device__ size_t star_lds(uint32_t L, uint32_t N, const dk_stars &st, 
     };
     char *K = take(sizeof(double) * L * L);
     char *bt = take(sizeof(uint32_t) * (L + 1) * (N + 2));
-    char *acc = take(sizeof(double) * st.selems);
-    char *ctr = take(sizeof(T) * st.selems);
-    char *ms = take(sizeof(uint32_t) * st.sgmax * st.ymax);
-    char *mz = take(sizeof(uint32_t) * st.sgmax * st.ymax);
+    // centres: + kCtrPad entries behind the staged ones (zero values / scratch
+    // accumulators for lanes without a centre hop); member tables: + one row
+    // of absent members (the buffer-addressed column pass)
+    char *acc = take(sizeof(double) * (st.selems + kCtrPad));
+    char *ctr = take(sizeof(T) * (st.selems + kCtrPad));
+    char *ms = take(sizeof(uint32_t) * (st.sgmax + 1) * st.ymax);
+    char *mz = take(sizeof(uint32_t) * (st.sgmax + 1) * st.ymax);
     char *cs = take(sizeof(uint32_t) * st.sgmax);
     char *cz = take(sizeof(uint32_t) * st.sgmax);
     char *yc = take(st.sgmax * (st.ymax > 8 ? st.ymax : 8)); // also the stars' A-parts (column pass)
@@ -107,12 +121,13 @@ __host__ __device__ size_t star_lds(uint32_t L, uint32_t N, const dk_stars &st, 
 }
 
 // Main pass LDS: K restricted to B x B, binomials C(x, j) for x <= b, the
-// staged slabs.  Small, so that many workgroups share a CU.
+// staged slabs (FP64 in either precision), the group's slab starts and slot
+// bases.
 template <typename T>
 __host__ __device__ size_t main_lds(uint32_t L, uint32_t N, const dk_stars &st)
 {
     return a8(sizeof(double) * (st.b * st.b + 1)) + a8(sizeof(uint32_t) * (st.b + 1) * (N + 2)) +
-           a8(sizeof(T) * st.emax) + 2 * a8(sizeof(uint32_t) * st.emax);
+           a8(sizeof(double) * st.owns) + 2 * a8(sizeof(uint32_t) * st.gslabs);
 }
 
 // One lane per (star, member offset beta).  The lane loads the member slabs'
@@ -130,6 +145,7 @@ __host__ __device__ size_t main_lds(uint32_t L, uint32_t N, const dk_stars &st)
 struct StarCtx {
     int L, N, a, b;
     uint32_t n, ztot;
+    uint32_t zm_sector, zm_len; // first Z index and length of the members' sector
     int nst, j, mc, mm, Sm, Sc, ny;
     bool has_ctr;
     const uint16_t *ins; // insert table of the members' B-pair count
@@ -140,7 +156,11 @@ struct StarCtx {
 // kByY (the column pass): no K, and the member tables indexed by orbital,
 // entry s * a + y for every y < a (zero for y in rho), copied from the
 // host-built st.mtab instead of computed by member number s * ny + yi.
-template <typename T, bool kChecked, int NT = kBlock, bool kByY = false, typename OK>
+// kBuf (with kByY): the member tables hold byte offsets instead, the first
+// rank times sizeof(T) and the slot position relative to the members' Z
+// sector times sizeof(T), kOob for y in rho and in the extra row nst; the
+// kCtrPad centre values behind the staged ones are zero.
+template <typename T, bool kChecked, int NT = kBlock, bool kByY = false, bool kBuf = false, typename OK>
 __device__ __forceinline__ StarCtx star_setup(const dk_tables &t, const dk_stars &st, uint32_t sg,
                                               const T *__restrict__ c, StarLds &p, OK &&ok)
 {
@@ -176,6 +196,8 @@ __device__ __forceinline__ StarCtx star_setup(const dk_tables &t, const dk_stars
     x.ins = st.ins + st.ins_off[has_ctr ? mm : 0];
     const int km = j + 1;
     const uint32_t zm_sector = st.zoff[km], zc_sector = st.zoff[j];
+    x.zm_sector = zm_sector;
+    x.zm_len = st.zoff[km + 1] - zm_sector;
     const int nsm = nslot_of(km, mm, a), nsc = nslot_of(j, mc, a);
     const uint64_t amask_all = a < 64 ? ((1ull << a) - 1) : ~0ull;
 
@@ -221,7 +243,21 @@ __device__ __forceinline__ StarCtx star_setup(const dk_tables &t, const dk_stars
     ctr_load(0);
 
     // member tables: first rank, Z position of this star's slot, orbital y
-    if (kByY) { // built on the host (st.mtab), contiguous for the group's stars
+    if (kByY && kBuf) { // byte offsets, kOob for absent members
+        const uint2 *mt = (const uint2 *)st.mtab + (size_t)first * a;
+        for (int task = threadIdx.x; task < (nst + 1) * a; task += NT) {
+            const int s = task / a, y = task - s * a;
+            uint32_t o0 = kOob, o1 = kOob;
+            if (s < nst && !((srho[s] >> y) & 1)) {
+                const uint2 v = mt[task];
+                o0 = v.x * (uint32_t)sizeof(T);
+                o1 = (v.y - zm_sector) * (uint32_t)sizeof(T);
+            }
+            p.mem_start[task] = o0;
+            p.mem_zb[task] = o1;
+        }
+        if (threadIdx.x < kCtrPad) ((T *)p.ctr)[st.selems + threadIdx.x] = (T)0;
+    } else if (kByY) { // built on the host (st.mtab), contiguous for the group's stars
         const uint2 *mt = (const uint2 *)st.mtab + (size_t)first * a;
         for (int task = threadIdx.x; task < nst * a; task += NT) {
             const uint2 v = mt[task];
@@ -739,6 +775,216 @@ __global__ __launch_bounds__(kBlock, 3) void k_star_col(dk_tables t, dk_stars st
     star_finish<T>(x, p, Z, ok);
 }
 
+// AND of a 64-bit value over the 64 lanes of a wave (all lanes active), by
+// DPP row shifts and row broadcasts; the result is read from lane 63.
+__device__ __forceinline__ uint32_t wave_and32(uint32_t v)
+{
+    // row_shr:1, 2, 4, 8 (lanes without a source keep the AND identity)
+    v &= (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x111, 0xf, 0xf, false);
+    v &= (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x112, 0xf, 0xf, false);
+    v &= (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x114, 0xf, 0xf, false);
+    v &= (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x118, 0xf, 0xf, false);
+    // row_bcast:15 into rows 1 and 3, row_bcast:31 into rows 2 and 3
+    v &= (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x142, 0xa, 0xf, false);
+    v &= (uint32_t)__builtin_amdgcn_update_dpp(-1, (int)v, 0x143, 0xc, 0xf, false);
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+__device__ __forceinline__ uint64_t wave_and64(uint64_t v)
+{
+    return ((uint64_t)wave_and32((uint32_t)(v >> 32)) << 32) | wave_and32((uint32_t)v);
+}
+
+// Buffer loads and stores at a 32-bit byte offset (range-checked by the
+// hardware: an offset past the buffer reads 0 and drops a store).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void *base, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0, (int)bytes, 0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ T buf_load(__amdgpu_buffer_rsrc_t r, uint32_t off)
+{
+    if constexpr (sizeof(T) == 8)
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+    else
+        return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+template <typename T>
+__device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off, T v)
+{
+    if constexpr (sizeof(T) == 8) {
+        typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, v), r, (int)off, 0, 0);
+    } else {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, (int)off, 0, 0);
+    }
+}
+
+// Column star pass, buffer-addressed (the default where the vector and every
+// members' sector of Z are below kBufLimit bytes).  The same products as
+// k_star_col; what differs is the addressing:
+// * member values, insert offsets and slot stores go through buffer
+//   resources with 32-bit byte offsets (the vector; the group's insert-table
+//   rows, the row as a scalar offset; the members' Z sector).  An absent
+//   member (y in rho; every member of a lane past the last column, which
+//   reads the member-table row nst) has the offset kOob, so its load returns
+//   0 and its slot store is dropped by the range check: no per-load address
+//   selects and no exec-mask changes around the stores;
+// * a lane whose centre value does not exist (t in beta, or past the last
+//   column) reads and accumulates at its own entry behind the staged centres
+//   (value 0, accumulator never read), so the centre reads and the LDS
+//   atomics are unconditional;
+// * col / Sm from a float reciprocal and one correction step (col < 2^24);
+// * the orbitals in every lane's star: the AND of the A-parts of the stars
+//   the wave's columns touch, a scalar loop over the first and last lane's.
+template <typename T, bool kChecked, int AMAX, int BMAX, bool kExact>
+__global__ __launch_bounds__(kBlock, DK_COLB_WAVES) void k_star_colb(dk_tables t, dk_stars st, const T *__restrict__ c,
+                                                         T *__restrict__ Z, uint32_t *bad,
+                                                         const dk_sigma_io *io,
+                                                         const double *__restrict__ KP)
+{
+    constexpr int R = AMAX + BMAX;
+    extern __shared__ double lds_raw[];
+    const uint32_t sg = blockIdx.x;
+    if (sg >= st.nsgroup) return;
+    if (io) {
+        if (!io->active) return;
+        c = (const T *)io->c;
+    }
+    StarLds p;
+    star_lds<T>(t.L, t.N, st, &p, (char *)lds_raw);
+    auto ok = [&](uint32_t idx, uint32_t lim) -> bool {
+        if (!kChecked) return true;
+        if (idx < lim) return true;
+        atomicAdd(bad, 1u);
+        return false;
+    };
+    const StarCtx x = star_setup<T, kChecked, kBlock, true, true>(t, st, sg, c, p, ok);
+    const int a = (int)st.a, b = (int)st.b;
+    const int Sm = __builtin_amdgcn_readfirstlane(x.Sm);
+    const int Sc = __builtin_amdgcn_readfirstlane(x.Sc);
+    const int nst = __builtin_amdgcn_readfirstlane(x.nst);
+    const bool has_ctr = __builtin_amdgcn_readfirstlane((int)x.has_ctr) != 0;
+    if (a > AMAX || b > BMAX || (kExact && a != AMAX)) { // uniform: the launcher picks AMAX >= a, BMAX >= b
+        if (kChecked && threadIdx.x == 0) atomicAdd(bad, 1u);
+        return;
+    }
+    const uint32_t cbytes = x.n * (uint32_t)sizeof(T), zbytes = x.zm_len * (uint32_t)sizeof(T);
+    const __amdgpu_buffer_rsrc_t rc = buf_rsrc(c, cbytes);
+    const __amdgpu_buffer_rsrc_t rz = buf_rsrc(Z + x.zm_sector, zbytes);
+    const __amdgpu_buffer_rsrc_t ri = buf_rsrc(x.ins, has_ctr ? (uint32_t)(2 * b * Sm) : 0u);
+    const int ncol = nst * Sm;
+    const float inv = 1.0f / (float)Sm;
+    const uint64_t amask = a < 64 ? ((1ull << a) - 1) : ~0ull;
+    const int lane = (int)(threadIdx.x & 63);
+    const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const T *ctrv = (const T *)p.ctr;
+    const int dummy = (int)st.selems + lane;
+    for (int c0 = wave * 64; c0 < ncol; c0 += kBlock) {
+        const double *kp = KP + opaque(0, c0);
+        const int col = c0 + lane;
+        const bool live = col < ncol;
+        const int colc = live ? col : ncol - 1;
+        int sq = (int)((float)colc * inv);
+        int beta = colc - (int)__umul24(sq, Sm);
+        if (beta < 0) {
+            --sq;
+            beta += Sm;
+        } else if (beta >= Sm) {
+            ++sq;
+            beta -= Sm;
+        }
+        // orbitals in rho for every lane's star (AND over the wave, lanes past
+        // the last column repeat its star): skipped by the whole wave
+        const uint64_t rho_l = ((const uint64_t *)p.ycol)[sq];
+        const uint64_t common = wave_and64(rho_l) & amask;
+        if (kChecked) { // the same AND by lane shuffles
+            uint32_t clo = (uint32_t)rho_l, chi = (uint32_t)(rho_l >> 32);
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) {
+                clo &= (uint32_t)__shfl_xor((int)clo, o);
+                chi &= (uint32_t)__shfl_xor((int)chi, o);
+            }
+            if (((((uint64_t)chi << 32) | clo) & amask) != common) atomicAdd(bad, 1u);
+        }
+        const uint64_t ym = ~common & amask; // the wave's orbitals y
+        const int s = live ? sq : nst;       // row nst: every member absent
+        const uint32_t bb = live ? (uint32_t)beta * (uint32_t)sizeof(T) : 0u;
+        const uint32_t *ms = p.mem_start + __umul24(s, a);
+        const uint32_t *mz = p.mem_zb + __umul24(s, a);
+        // member values, all loads in flight at once (an orbital common to
+        // the wave has the offset kOob in every lane: no memory access)
+        T v[AMAX];
+#pragma unroll
+        for (int y = 0; y < AMAX; ++y) {
+            // (padded rows y >= a: no member, no table entry)
+            const uint32_t o = (kExact || y < a) ? ms[y] + bb : kOob;
+            if (kChecked && o < kOob) ok(o, cbytes);
+            v[y] = buf_load<T>(rc, o);
+        }
+        // the centre's insert offsets lexB(beta + t), loaded while the member
+        // values arrive and the A products run
+        uint32_t offs[BMAX];
+        if (has_ctr) {
+            const int bq = opaque(b, c0), Smq = opaque(Sm, c0);
+            const int vo = 2 * beta;
+#pragma unroll
+            for (int q = 0; q < BMAX; ++q)
+                offs[q] = q < bq ? (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(ri, vo, 2 * q * Smq, 0)
+                                 : 0xffffu;
+        }
+        double out[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) out[r] = 0.0;
+        const double *kctr = kp + AMAX * R; // centre columns
+        double kbuf[kKC];
+        {
+            const double *k0 = ym ? kp + (__ffsll((unsigned long long)ym) - 1) * R : kctr;
+#pragma unroll
+            for (int i = 0; i < kKC; ++i) kbuf[i] = k0[i];
+        }
+#pragma unroll
+        for (int y = 0; y < AMAX; ++y) {
+            if ((ym >> y) & 1) {
+                const uint64_t rest = ym & ~((2ull << y) - 1);
+                const double *kn = rest ? kp + (__ffsll((unsigned long long)rest) - 1) * R : kctr;
+                col_fma<R>(kp + y * R, kn, (double)v[y], out, kbuf);
+            }
+        }
+        if (has_ctr) {
+            const int bq = opaque(b, c0);
+            const int cb = (int)__umul24(s, Sc);
+            int ci[BMAX];
+#pragma unroll
+            for (int q = 0; q < BMAX; ++q) {
+                const bool o = live && offs[q] != 0xffffu && ok(offs[q], (uint32_t)Sc);
+                ci[q] = o ? cb + (int)offs[q] : dummy;
+            }
+            double cv[BMAX];
+#pragma unroll
+            for (int q = 0; q < BMAX; ++q) cv[q] = (double)ctrv[ci[q]];
+            double *acc = p.acc;
+#pragma unroll
+            for (int q = 0; q < BMAX; ++q)
+                __hip_atomic_fetch_add(&acc[ci[q]], out[AMAX + q], __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_WORKGROUP);
+#pragma unroll
+            for (int q = 0; q < BMAX; ++q)
+                if (q < bq)
+                    col_fma<AMAX>(kctr + q * AMAX, q + 1 < bq ? kctr + (q + 1) * AMAX : kp, cv[q], out, kbuf);
+        }
+        // member slots (absent members: dropped by the range check)
+#pragma unroll
+        for (int xo = 0; xo < AMAX; ++xo) {
+            if (xo < a && !((common >> xo) & 1)) {
+                const uint32_t o = mz[xo] + bb;
+                if (kChecked && o < kOob) ok(o, zbytes);
+                buf_store<T>(rz, o, (T)out[xo]);
+            }
+        }
+    }
+    star_finish<T>(x, p, Z, ok);
+}
+
 // FP32 star pass on the matrix cores.  The column formulation of
 // k_star_col (products over all of A, K independent of the star) with
 // v_mfma_f32_16x16x4f32 tiles: a wave takes 16 columns at a time; the B
@@ -890,6 +1136,109 @@ __global__ __launch_bounds__(kBlockM) void k_star_m32(dk_tables t, dk_stars st, 
     star_finish<float, kBlockM>(x, p, Z, ok);
 }
 
+// Main pass over one set of SPL slabs of a group, lane per offset bi: the
+// hop table entry of (h, bi) and its K are read once for all SPL slabs, whose
+// values sit side by side in LDS (own[(set * S + bi) * main_stride(SPL) + u],
+// FP64; the stride is padded to an odd number of doubles for SPL > 1, so that
+// the lanes' random offsets J spread over the banks as for a single slab), so
+// a hop costs one table load, one K read and an address per set plus one LDS
+// read and one FMA per slab.  Per determinant the sums are added in the same
+// order as for a single slab: ((D - shift) c + hops in table order) + slots.
+__host__ __device__ constexpr int main_stride(int spl) { return spl == 1 ? 1 : spl + 1; }
+// slabs per set of a main-pass group of nsl slabs (groups hold a multiple of
+// 4 or 2 slabs except at the end of a sector)
+__host__ __device__ constexpr int main_spl(int nsl) { return (nsl & 3) == 0 ? 4 : (nsl & 1) == 0 ? 2 : 1; }
+
+template <int SPL, typename T, bool kChecked, typename OK>
+__device__ __forceinline__ void main_sets(const double *__restrict__ D, double shift,
+                                          const T *__restrict__ Z, T *__restrict__ sigma,
+                                          const double *own, const uint32_t *gstart,
+                                          const uint32_t *gzb, const char *Kbb,
+                                          __amdgpu_buffer_rsrc_t hrsrc, int S, int nsets, int ns,
+                                          int nhp, uint32_t n, uint32_t ztot, uint32_t kbytes, OK &&ok)
+{
+    constexpr int kZS = SPL == 4 ? 8 : 12; // slots per slab loaded before the hops
+    constexpr int kHB = 8;                  // hop-table rows loaded together
+    for (int w = threadIdx.x; w < nsets * S; w += kBlock) {
+        const int set = w / S, bi = w - set * S;
+        constexpr int STR = main_stride(SPL);
+        const double *ob = own + (size_t)set * S * STR;
+        // the first hop-table rows, then the slots and the diagonal (HBM),
+        // all in flight together; each later chunk of rows is loaded while
+        // the one before is used
+        const uint32_t voff = (uint32_t)bi * 4u;
+        uint32_t hv[kHB];
+#pragma unroll
+        for (int i = 0; i < kHB; ++i)
+            hv[i] = __builtin_amdgcn_raw_buffer_load_b32(hrsrc, (int)voff, i * S * 4, 0);
+        uint32_t I[SPL];
+        double dI[SPL];
+        T zv[SPL][kZS];
+#pragma unroll
+        for (int u = 0; u < SPL; ++u) {
+            const int s = set * SPL + u;
+            I[u] = gstart[s] + (uint32_t)bi;
+            const uint32_t zb = gzb[s] + (uint32_t)bi;
+            // loads without per-slot branches, so that all of them are in
+            // flight together: slots past ns read the last one again, a slab
+            // without slots (a = 0) reads Z[0] (every slot buffer has an
+            // entry), and neither is summed
+#pragma unroll
+            for (int q = 0; q < kZS; ++q) {
+                const uint32_t zi = ns > 0 ? zb + (uint32_t)((q < ns ? q : ns - 1) * S) : 0u;
+                zv[u][q] = (ns == 0 || ok(zi, ztot)) ? Z[zi] : (T)0;
+            }
+            dI[u] = ok(I[u], n) ? D[I[u]] : 0.0;
+        }
+        double acc[SPL];
+#pragma unroll
+        for (int u = 0; u < SPL; ++u) acc[u] = 0.0;
+        for (int h0 = 0; h0 < nhp; h0 += kHB) {
+            uint32_t hn[kHB];
+            if (h0 + kHB < nhp) { // uniform
+#pragma unroll
+                for (int i = 0; i < kHB; ++i)
+                    hn[i] = __builtin_amdgcn_raw_buffer_load_b32(hrsrc, (int)voff, (h0 + kHB + i) * S * 4, 0);
+            }
+            // four hops at a time: every LDS read issued before the products
+#pragma unroll
+            for (int i0 = 0; i0 < kHB; i0 += 4) {
+                double Kv[4], ov[4][SPL];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    // J, and the byte offset of K in Kbb (which starts the LDS)
+                    const uint32_t J = hv[i0 + i] & 0xffffu, kb = hv[i0 + i] >> 16;
+                    const bool g = ok(J, (uint32_t)S) && ok(kb, kbytes);
+                    const double *o = (const double *)((const char *)ob + (g ? J : 0u) * (8u * STR));
+                    Kv[i] = g ? *(const double *)(Kbb + kb) : 0.0;
+#pragma unroll
+                    for (int u = 0; u < SPL; ++u) ov[i][u] = o[u];
+                }
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int i = 0; i < 4; ++i)
+#pragma unroll
+                    for (int u = 0; u < SPL; ++u) acc[u] += Kv[i] * ov[i][u];
+            }
+#pragma unroll
+            for (int i = 0; i < kHB; ++i) hv[i] = hn[i];
+        }
+#pragma unroll
+        for (int u = 0; u < SPL; ++u) {
+            const int s = set * SPL + u;
+            double zs = 0;
+#pragma unroll
+            for (int q = 0; q < kZS; ++q) zs += q < ns ? (double)zv[u][q] : 0.0;
+            const uint32_t zb = gzb[s] + (uint32_t)bi;
+            for (int q = kZS; q < ns; ++q) {
+                const uint32_t zi = zb + (uint32_t)(q * S);
+                if (ok(zi, ztot)) zs += (double)Z[zi];
+            }
+            if (ok(I[u], n)) sigma[I[u]] = (T)(((dI[u] - shift) * ob[bi * STR + u] + acc[u]) + zs);
+        }
+    }
+}
+
 template <typename T, bool kChecked>
 __global__ __launch_bounds__(kBlock) void k_star_main(dk_tables t, dk_stars st,
                                                       const double *__restrict__ D, double shift,
@@ -917,10 +1266,10 @@ __global__ __launch_bounds__(kBlock) void k_star_main(dk_tables t, dk_stars st,
     };
     double *Kbb = lds_raw; // Kbb[o * b + t] = K between B orbitals o and t; Kbb[b * b] = 0
     uint32_t *bt = (uint32_t *)((char *)lds_raw + a8(sizeof(double) * (b * b + 1)));
-    T *own = (T *)((char *)bt + a8(sizeof(uint32_t) * (b + 1) * (N + 2)));
-    // the group's slab starts and slot bases (a group has at most emax slabs)
-    uint32_t *gstart = (uint32_t *)((char *)own + a8(sizeof(T) * st.emax));
-    uint32_t *gzb = gstart + a8(sizeof(uint32_t) * st.emax) / sizeof(uint32_t);
+    double *own = (double *)((char *)bt + a8(sizeof(uint32_t) * (b + 1) * (N + 2)));
+    // the group's slab starts and slot bases
+    uint32_t *gstart = (uint32_t *)((char *)own + a8(sizeof(double) * st.owns));
+    uint32_t *gzb = gstart + a8(sizeof(uint32_t) * st.gslabs) / sizeof(uint32_t);
     for (int i = threadIdx.x; i <= b * b; i += kBlock) {
         const int o = i / b, tt = i - o * b;
         Kbb[i] = i < b * b ? t.K[(a + o) * L + a + tt] : 0.0;
@@ -928,6 +1277,10 @@ __global__ __launch_bounds__(kBlock) void k_star_main(dk_tables t, dk_stars st,
     for (int i = threadIdx.x; i < (b + 1) * (N + 2); i += kBlock) bt[i] = t.binom[i];
     const uint32_t first = st.group[2 * g], w = st.group[2 * g + 1];
     const int nsl = (int)(w & 0xffffu), m = (int)(w >> 16), k = N - m;
+    if (kChecked && (uint32_t)nsl > st.gslabs) {
+        if (threadIdx.x == 0) atomicAdd(bad, 1u);
+        return;
+    }
     __syncthreads();
     for (int i = threadIdx.x; i < nsl; i += kBlock) {
         gstart[i] = st.slab_start[first + i];
@@ -936,80 +1289,50 @@ __global__ __launch_bounds__(kBlock) void k_star_main(dk_tables t, dk_stars st,
     __syncthreads();
     const dk::Binom C{bt, N + 2};
     const int S = (int)C(b, m), nel = nsl * S, ns = nslot_of(k, m, a);
-    // stage the own slabs: four loads in flight per lane
+    const int spl = main_spl(nsl), str = main_stride(spl);
+    const int lsp = spl == 4 ? 2 : spl == 2 ? 1 : 0;
+    if (kChecked && (uint32_t)(nel / spl * str) > st.owns) {
+        if (threadIdx.x == 0) atomicAdd(bad, 1u);
+        return;
+    }
+    // stage the slabs as FP64, the values of a set side by side (element e:
+    // u = e % spl, then bi, then set; at r * str + u); four loads in flight
     for (int e0 = threadIdx.x; e0 < nel; e0 += 4 * kBlock) {
-        T v[4];
+        double v[4];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            const int e = e0 + u * kBlock;
-            v[u] = (T)0;
+        for (int q = 0; q < 4; ++q) {
+            const int e = e0 + q * kBlock;
+            v[q] = 0.0;
             if (e < nel) {
-                const int s = e / S, bi = e - s * S;
-                const uint32_t gi = gstart[s] + (uint32_t)bi;
-                if (ok(gi, n)) v[u] = c[gi];
+                const int u = e & (spl - 1), r = e >> lsp, set = r / S, bi = r - set * S;
+                const uint32_t gi = gstart[set * spl + u] + (uint32_t)bi;
+                if (ok(gi, n)) v[q] = (double)c[gi];
             }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (e0 + u * kBlock < nel) own[e0 + u * kBlock] = v[u];
+        for (int q = 0; q < 4; ++q) {
+            const int e = e0 + q * kBlock;
+            if (e < nel) own[(e >> lsp) * str + (e & (spl - 1))] = v[q];
+        }
     }
     __syncthreads();
-    constexpr int kZS = 16; // slots loaded together (more are summed by a tail loop)
-    constexpr int kHB = 8;  // hop-table rows loaded together (rows are padded to a multiple)
+    constexpr int kHB = 8;
     const int nhp = (m * (b - m) + kHB - 1) / kHB * kHB;
     // the hop table of m-pair masks, nhp rows of S entries (loads past it return 0)
     const __amdgpu_buffer_rsrc_t hrsrc = __builtin_amdgcn_make_buffer_rsrc(
         (void *)(st.hop + st.hop_off[m]), (short)0, nhp * S * 4, 0x00020000);
-    for (int e = threadIdx.x; e < nel; e += kBlock) {
-        const int s = e / S, bi = e - s * S;
-        const uint32_t I = gstart[s] + (uint32_t)bi;
-        if (!ok(I, n)) continue;
-        const T *os = own + s * S;
-        // the slots and the diagonal first (HBM), all in flight together
-        const uint32_t zb = gzb[s] + (uint32_t)bi;
-        T zv[kZS];
-#pragma unroll
-        for (int q = 0; q < kZS; ++q) {
-            zv[q] = (T)0;
-            if (q < ns) {
-                const uint32_t zi = zb + (uint32_t)(q * S);
-                if (ok(zi, ztot)) zv[q] = Z[zi];
-            }
-        }
-        const double dI = D[I];
-        // hops inside B from the hop table (coalesced across lanes), 8 rows at
-        // a time through a buffer resource: the row offset is a scalar, the
-        // lane's column offset computed once, padding rows add 0
-        const uint32_t voff = (uint32_t)bi * 4u;
-        double acc = 0;
-        for (int h0 = 0; h0 < nhp; h0 += kHB) {
-            uint32_t hv[kHB];
-#pragma unroll
-            for (int i = 0; i < kHB; ++i)
-                hv[i] = __builtin_amdgcn_raw_buffer_load_b32(hrsrc, (int)voff, (h0 + i) * S * 4, 0);
-#pragma unroll
-            for (int i = 0; i < kHB; ++i) {
-                // J, and the byte offset of K in Kbb (which starts the LDS)
-                const uint32_t J = hv[i] & 0xffffu, kb = hv[i] >> 16;
-                if (ok(J, (uint32_t)S) && ok(kb, (uint32_t)(8 * (b * b + 1))))
-                    acc += *(const double *)((const char *)Kbb + kb) * (double)os[J];
-            }
-        }
-        // (slots past ns hold zeros: the second half is skipped when the
-        // group has at most kZS / 2 slots, a uniform branch)
-        double zs = 0;
-#pragma unroll
-        for (int q = 0; q < kZS / 2; ++q) zs += (double)zv[q];
-        if (ns > kZS / 2) {
-#pragma unroll
-            for (int q = kZS / 2; q < kZS; ++q) zs += (double)zv[q];
-        }
-        for (int q = kZS; q < ns; ++q) {
-            const uint32_t zi = zb + (uint32_t)(q * S);
-            if (ok(zi, ztot)) zs += (double)Z[zi];
-        }
-        sigma[I] = (T)(((dI - shift) * (double)os[bi] + acc) + zs);
-    }
+    const int nsets = nsl >> lsp;
+    const char *Kb = (const char *)Kbb;
+    const uint32_t kbytes = (uint32_t)(8 * (b * b + 1));
+    if (spl == 4)
+        main_sets<4, T, kChecked>(D, shift, Z, sigma, own, gstart, gzb, Kb, hrsrc, S, nsets, ns, nhp,
+                                  n, ztot, kbytes, ok);
+    else if (spl == 2)
+        main_sets<2, T, kChecked>(D, shift, Z, sigma, own, gstart, gzb, Kb, hrsrc, S, nsets, ns, nhp,
+                                  n, ztot, kbytes, ok);
+    else
+        main_sets<1, T, kChecked>(D, shift, Z, sigma, own, gstart, gzb, Kb, hrsrc, S, nsets, ns, nhp,
+                                  n, ztot, kbytes, ok);
 }
 
 } // namespace
@@ -1019,12 +1342,13 @@ __global__ __launch_bounds__(kBlock) void k_star_main(dk_tables t, dk_stars st,
 // padded to a multiple of 8 and b to 16: fewer FMAs, loads, stores and
 // registers.  Instantiated in sigma_star_exact_lo.hip (a = 12..16) and
 // sigma_star_exact_hi.hip (a = 17..21), separate translation units that
-// compile in parallel.  Each launches k_kpack and k_star_col on stream s and
-// returns 0, or returns -1 when it has no instantiation for (a, b).
+// compile in parallel.  Each launches k_kpack and k_star_colb (buf != 0) or
+// k_star_col on stream s and returns 0, or returns -1 when it has no
+// instantiation for (a, b).
 __host__ __device__ constexpr bool col_exact_shape(int a, int b) { return b == 13 && a >= 12 && a <= 21; }
-int dk_col_exact_lo(int prec, int checked, int a, int b, dim3 grid, size_t lds, hipStream_t s,
+int dk_col_exact_lo(int prec, int checked, int buf, int a, int b, dim3 grid, size_t lds, hipStream_t s,
                     const dk_tables &t, const dk_stars &st, const void *c, void *Z, uint32_t *bad,
                     const dk_sigma_io *io, double *kp);
-int dk_col_exact_hi(int prec, int checked, int a, int b, dim3 grid, size_t lds, hipStream_t s,
+int dk_col_exact_hi(int prec, int checked, int buf, int a, int b, dim3 grid, size_t lds, hipStream_t s,
                     const dk_tables &t, const dk_stars &st, const void *c, void *Z, uint32_t *bad,
                     const dk_sigma_io *io, double *kp);
