@@ -44,6 +44,7 @@ int launch(const dk_tables *t, const dk_stars *st, const double *D, double shift
         // (buffer-addressed where the vector and every members' Z sector fit 32-bit offsets)
         double *kp = (double *)((char *)Z + sizeof(double) * (size_t)st->zoff[a + 1]);
         const dim3 grid(st->nsgroup);
+        const size_t ls = star_lds<T, false>(t->L, t->N, *st, nullptr, nullptr); // no K in LDS
         const int buf = st->pass != 4 && (uint64_t)t->n * sizeof(T) < kBufLimit &&
                         (uint64_t)st->zsec_max * sizeof(T) < kBufLimit;
         int done = -1;

@@ -85,7 +85,9 @@ struct StarLds {
 
 // Star pass LDS: K, binomials, per star the centre slab (values + FP64
 // accumulator, together `selems` elements per group) and member tables.
-template <typename T>
+// kWithK = false (the column passes, which read K through scalar loads):
+// no K table, so that more workgroups fit a CU.
+template <typename T, bool kWithK = true>
 __host__ __device__ size_t star_lds(uint32_t L, uint32_t N, const dk_stars &st, StarLds *p, char *base)
 {
     size_t off = 0;
@@ -94,7 +96,7 @@ __host__ __device__ size_t star_lds(uint32_t L, uint32_t N, const dk_stars &st, 
         off += a8(bytes);
         return q;
     };
-    char *K = take(sizeof(double) * L * L);
+    char *K = take(kWithK ? sizeof(double) * L * L : 0);
     char *bt = take(sizeof(uint32_t) * (L + 1) * (N + 2));
     // centres: + kCtrPad entries behind the staged ones (zero values / scratch
     // accumulators for lanes without a centre hop); member tables: + one row
@@ -639,7 +641,7 @@ __global__ __launch_bounds__(kBlock, 3) void k_star_col(dk_tables t, dk_stars st
         c = (const T *)io->c;
     }
     StarLds p;
-    star_lds<T>(t.L, t.N, st, &p, (char *)lds_raw);
+    star_lds<T, false>(t.L, t.N, st, &p, (char *)lds_raw);
     auto ok = [&](uint32_t idx, uint32_t lim) -> bool {
         if (!kChecked) return true;
         if (idx < lim) return true;
@@ -851,7 +853,7 @@ __global__ __launch_bounds__(kBlock, DK_COLB_WAVES) void k_star_colb(dk_tables t
         c = (const T *)io->c;
     }
     StarLds p;
-    star_lds<T>(t.L, t.N, st, &p, (char *)lds_raw);
+    star_lds<T, false>(t.L, t.N, st, &p, (char *)lds_raw);
     auto ok = [&](uint32_t idx, uint32_t lim) -> bool {
         if (!kChecked) return true;
         if (idx < lim) return true;
