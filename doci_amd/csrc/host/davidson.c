@@ -6,29 +6,35 @@
  * vectors, and the returned code its >= max_its.
  *
  * MI355X form: the subspace V and its images W = (H - shift) V live in HBM;
- * each iteration is four streaming passes (orthogonalise, sigma, one new
- * column of V^T W, fused Ritz/residual/preconditioner) and only O(m)
- * scalars cross to the host.  Vectors are stored unnormalised with a scale
- * per vector (V_j = s_j Vt_j), which removes the normalisation pass.  The
- * orthogonalisation is classical Gram-Schmidt repeated while the measured
- * loss of orthogonality exceeds a threshold (the reference uses modified
- * Gram-Schmidt; both give the same subspace to rounding).  The Rayleigh-Ritz
- * step solves S y = theta G y with the Gram matrix G = V^T V measured in FP64,
- * which equals I for FP64 storage and absorbs the rounding of FP32 storage.
+ * each iteration is three streaming passes (sigma, one new column of V^T W,
+ * fused Ritz/residual/preconditioner) and only O(m) scalars cross to the
+ * host.  Vectors are stored unnormalised with a scale per vector
+ * (V_j = s_j Vt_j), which removes the normalisation pass.  The reference
+ * orthogonalises each correction against V (modified Gram-Schmidt); here a
+ * correction joins the basis as it is and the Rayleigh-Ritz step solves
+ * S y = theta G y with the Gram matrix G = V^T V measured in FP64 (the same
+ * subspace, so the same Ritz pairs to rounding); a correction nearly
+ * dependent on V is projected out explicitly first.  G also absorbs the
+ * rounding of FP32 storage.
  *
  * Mixed precision (precision = 1): the subspace is stored in FP32 with FP64
  * sums and the operator applied as H - shift with shift = min(D) (so stored
  * W entries are small and FP32 rounding of W stays below the switch
  * tolerance); once the residual drops below switch_tol the solve restarts in
  * FP64 from the current Ritz vector and finishes with FP64 residuals and
- * Rayleigh quotients.  The restart keeps the FP32 phase's last correction
- * (its preconditioned residual, as an FP64 vector): the first FP64 iteration
- * computes the FP64 sigma of the Ritz vector and then orthogonalises that
- * correction, instead of recomputing the residual of the same vector.  The
- * FP64 phase always runs after an FP32 phase, also when the FP32 residual is
- * already below tol (switch_tol <= tol, or one iteration past both): an FP32
- * residual describes the rounded operator, so convergence is only reported
- * from an FP64 sigma, residual and Rayleigh quotient.
+ * Rayleigh quotients.  The FP32 phase also hands over when its residual has
+ * stopped falling (not halved over the last four iterations): FP32 storage
+ * bounds the residual it can reach (a few times 1e-8 on the reference's
+ * inputs), and a switch_tol below that would otherwise spend every remaining
+ * iteration in FP32.  The restart keeps the FP32 phase's last correction
+ * (its preconditioned residual, as an FP64 vector) when that phase met
+ * switch_tol: the first FP64 iteration computes the FP64 sigma of the Ritz
+ * vector and then adds that correction to the basis, instead of recomputing
+ * the residual of the same vector.  The FP64 phase always runs after an FP32
+ * phase, also when the FP32 residual is already below tol (switch_tol <= tol,
+ * or one iteration past both): an FP32 residual describes the rounded
+ * operator, so convergence is only reported from an FP64 sigma, residual and
+ * Rayleigh quotient.
  */
 #include <math.h>
 #include <stdio.h>
@@ -36,6 +42,8 @@
 #include <string.h>
 
 #include "internal.h"
+
+#define DK_MAX_STALL 1024 /* FP32 phase iterations watched for a stalled residual */
 
 typedef struct {
     int prec;
@@ -152,7 +160,7 @@ static int dav_phase(dav_run *r, int prec, double shift, double stop_tol, double
     const doci_solve_opts *o = r->o;
     const int mv = o->davidson_max_vec, keep = o->davidson_keep, ld = mv;
     const int64_t n = c->n;
-    const double ortho_tol = prec ? 1e-6 : 1e-12;
+    const double dep_tol = 0.05;
     dav_ws w;
     int rc = ws_alloc(&w, c, n, prec, mv + keep);
     if (rc) return rc;
@@ -163,11 +171,13 @@ static int dav_phase(dav_run *r, int prec, double shift, double stop_tol, double
     void **Vp = calloc((size_t)mv + 1, sizeof(void *)), **Wp = calloc((size_t)mv + 1, sizeof(void *));
     /* P: the projected matrix of the shifted operator in the stored vectors,
      * P[i + j ld] = <V_i, W_j> (i <= j, unscaled); gam: the coefficients the
-     * orthogonalisation subtracted from t; bw: <W_j, t> from the residual */
+     * projection subtracted from t (none unless t was nearly dependent on V); bw: <W_j, t> from the residual */
     double *P = calloc((size_t)mv * mv, sizeof(double)), *gam = calloc((size_t)mv + 2, sizeof(double));
     double *bw = calloc((size_t)mv + 2, sizeof(double));
+    double *bv = calloc((size_t)mv + 2, sizeof(double)), *cv = calloc((size_t)mv + 2, sizeof(double));
     int have_b = 0;
-    if (!S || !G || !Y || !th || !s || !a || !red || !g || !Vp || !Wp || !P || !gam || !bw) {
+    if (!S || !G || !Y || !th || !s || !a || !red || !g || !Vp || !Wp || !P || !gam || !bw || !bv ||
+        !cv) {
         rc = doci_fail(DOCI_E_NOMEM, "Davidson host arrays");
         goto done;
     }
@@ -195,28 +205,36 @@ static int dav_phase(dav_run *r, int prec, double shift, double stop_tol, double
     tt = red[0];
     double res = stop_tol * 10, theta = 0; /* davidson.c:45: start above the tolerance */
     const int start_its = *r->its;
+    double hist[DK_MAX_STALL]; /* residuals of this phase's iterations */
     while (res > stop_tol && (*r->its)++ < o->max_its) {
-        /* orthogonalise t against V (davidson.c:133-176) */
+        /* the new direction t (davidson.c:133-176 orthogonalises it): it joins
+         * the basis as it is, the Rayleigh-Ritz step using the measured Gram
+         * matrix G of the stored vectors, unless it is nearly dependent on V
+         * (its part orthogonal to V below dep_tol of its norm squared); then
+         * its projection onto V, G^-1 <V, t>, is subtracted (at most twice).
+         * Consecutive corrections stay far from dependent (Gram condition
+         * numbers of a few units over a whole solve), so in practice no
+         * iteration pays a pass over V for this. */
         for (int j = 0; j < m; ++j) {
             Vp[j] = w.Vs[w.basis[j]];
             gam[j] = 0;
         }
-        for (int pass = 0; m > 0 && pass < 3; ++pass) {
+        for (int pass = 0; m > 0 && pass < 2; ++pass) {
+            for (int j = 0; j < m; ++j) bv[j] = s[j] * a[j];
+            const double pn = dh_gram_solve(m, G, ld, bv, cv);
+            if (pn < 0) {
+                rc = doci_fail(DOCI_E_BREAKDOWN, "Davidson: subspace Gram matrix not positive definite");
+                goto done;
+            }
+            if (!(tt > 0) || tt - pn > dep_tol * tt) break;
             for (int j = 0; j < m; ++j) {
-                g[j] = s[j] * s[j] * a[j];
+                g[j] = s[j] * cv[j];
                 gam[j] += g[j];
             }
             DK(dk_ortho(m, Vp, g, w.Vs[tslot], n, prec, c->d_red, c->d_out, c->stream));
             CK(dh_fetch(c, m + 1, red));
             tt = red[0];
-            double loss = 0;
-            for (int j = 0; j < m; ++j) {
-                a[j] = red[1 + j];
-                const double cj = fabs(s[j] * a[j]);
-                if (cj > loss) loss = cj;
-            }
-            if (!(tt > 0)) break;
-            if (loss <= ortho_tol * sqrt(tt)) break;
+            for (int j = 0; j < m; ++j) a[j] = red[1 + j];
         }
         if (!(tt > 0) || !isfinite(tt)) {
             rc = doci_fail(DOCI_E_BREAKDOWN, "Davidson: new direction has zero or invalid norm");
@@ -291,7 +309,7 @@ static int dav_phase(dav_run *r, int prec, double shift, double stop_tol, double
             CK(dh_gen_eig(m, S, G, ld, th, Y, ld));
         }
         theta = th[0];
-        /* residual, preconditioner and the dots for the next orthogonalisation */
+        /* residual, preconditioner and the dots of the next correction with V */
         for (int j = 0; j < m; ++j) {
             Vp[j] = w.Vs[w.basis[j]];
             Wp[j] = w.Ws[w.basis[j]];
@@ -299,8 +317,8 @@ static int dav_phase(dav_run *r, int prec, double shift, double stop_tol, double
         }
         tslot = take_slot(&w);
         if (t_in && m == 1 && *r->its == start_its + 1) {
-            /* the guess's correction is given: its overlaps for the next
-             * orthogonalisation (same subspace as recomputing it) */
+            /* the guess's correction is given: its overlaps with V (same
+             * subspace as recomputing it) */
             DK(dk_convert(t_in, 0, w.Vs[tslot], prec, n, 1.0, c->stream));
             DK(dk_multidot(m, Vp, w.Vs[tslot], n, prec, c->d_red, c->d_out, c->stream));
             CK(dh_fetch(c, m + 1, red));
@@ -326,6 +344,11 @@ static int dav_phase(dav_run *r, int prec, double shift, double stop_tol, double
         *energy = theta;
         if (prec) r->info->iterations_fp32++;
         if (o->trace) printf("%d\t%d\t%e\t%f\n", *r->its, 0, res, theta);
+        /* FP32 storage: hand over to FP64 once the residual has stopped
+         * falling (not halved over the last four iterations) */
+        const int k = *r->its - start_its - 1;
+        if (prec && k < DK_MAX_STALL) hist[k] = res;
+        if (prec && k >= 4 && k < DK_MAX_STALL && res > 0.5 * hist[k - 4]) break;
     }
     if (t_out && m > 0 && res <= stop_tol) DK(dk_convert(w.Vs[tslot], prec, t_out, 0, n, 1.0, c->stream));
     /* Ritz vector x = V y (davidson.c:205) */
@@ -357,6 +380,8 @@ done:
     free(P);
     free(gam);
     free(bw);
+    free(bv);
+    free(cv);
     return rc;
 }
 

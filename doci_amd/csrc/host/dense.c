@@ -74,6 +74,42 @@ void dh_jacobi(int m, double *A, int lda, double *w, double *Z, int ldz)
     free(ord);
 }
 
+/* c = G^-1 b for SPD G (upper triangle, column-major, leading dimension
+ * ld) by Cholesky; returns b^T c, or -1 when G is not positive definite. */
+double dh_gram_solve(int m, const double *G, int ld, const double *b, double *c)
+{
+    double *Lc = calloc((size_t)m * m, sizeof(double));
+    double r = -1;
+    if (!Lc) return -1;
+    for (int j = 0; j < m; ++j) {
+        double d = G[j + (size_t)j * ld];
+        for (int k = 0; k < j; ++k) d -= Lc[j + (size_t)k * m] * Lc[j + (size_t)k * m];
+        if (!(d > 0)) goto out;
+        d = sqrt(d);
+        Lc[j + (size_t)j * m] = d;
+        for (int i = j + 1; i < m; ++i) {
+            double v = G[j + (size_t)i * ld];
+            for (int k = 0; k < j; ++k) v -= Lc[i + (size_t)k * m] * Lc[j + (size_t)k * m];
+            Lc[i + (size_t)j * m] = v / d;
+        }
+    }
+    for (int i = 0; i < m; ++i) { /* Lc z = b */
+        double v = b[i];
+        for (int k = 0; k < i; ++k) v -= Lc[i + (size_t)k * m] * c[k];
+        c[i] = v / Lc[i + (size_t)i * m];
+    }
+    for (int i = m - 1; i >= 0; --i) { /* Lc^T c = z */
+        double v = c[i];
+        for (int k = i + 1; k < m; ++k) v -= Lc[k + (size_t)i * m] * c[k];
+        c[i] = v / Lc[i + (size_t)i * m];
+    }
+    r = 0;
+    for (int i = 0; i < m; ++i) r += b[i] * c[i];
+out:
+    free(Lc);
+    return r;
+}
+
 /* S y = w G y for symmetric S and SPD G (upper triangles used, column-major
  * with leading dimension ld); Y has G-orthonormal columns (Y^T G Y = I).
  * Returns DOCI_E_BREAKDOWN when G is not positive definite. */

@@ -84,6 +84,7 @@ int dh_op_from_tables(unsigned L, unsigned N, double core, const double *K, cons
 /* dense.c: small dense eigensolvers */
 void dh_jacobi(int m, double *A, int lda, double *w, double *Z, int ldz);
 int dh_gen_eig(int m, const double *S, const double *G, int ld, double *w, double *Y, int ldy);
+double dh_gram_solve(int m, const double *G, int ld, const double *b, double *c);
 int dh_sygv2(const double *A, const double *B, double *w, double *Z);
 
 /* Solver context: everything the device Davidson / CG need besides the
