@@ -246,17 +246,28 @@ __device__ __forceinline__ StarCtx star_setup(const dk_tables &t, const dk_stars
 
     // member tables: first rank, Z position of this star's slot, orbital y
     if (kByY && kBuf) { // byte offsets, kOob for absent members
+        // eight table entries per lane in flight, loaded without branches
+        // (entries of the extra row read entry 0 and are not used)
         const uint2 *mt = (const uint2 *)st.mtab + (size_t)first * a;
-        for (int task = threadIdx.x; task < (nst + 1) * a; task += NT) {
-            const int s = task / a, y = task - s * a;
-            uint32_t o0 = kOob, o1 = kOob;
-            if (s < nst && !((srho[s] >> y) & 1)) {
-                const uint2 v = mt[task];
-                o0 = v.x * (uint32_t)sizeof(T);
-                o1 = (v.y - zm_sector) * (uint32_t)sizeof(T);
+        constexpr int kML = 8;
+        const int ntask = (nst + 1) * a;
+        for (int t0 = threadIdx.x; t0 < ntask; t0 += kML * NT) {
+            uint2 v[kML];
+#pragma unroll
+            for (int q = 0; q < kML; ++q) {
+                const int task = t0 + q * NT;
+                v[q] = mt[task < nst * a ? task : 0];
             }
-            p.mem_start[task] = o0;
-            p.mem_zb[task] = o1;
+#pragma unroll
+            for (int q = 0; q < kML; ++q) {
+                const int task = t0 + q * NT;
+                if (task < ntask) {
+                    const int s = task / a, y = task - s * a;
+                    const bool mem = s < nst && !((srho[s < nst ? s : 0] >> y) & 1);
+                    p.mem_start[task] = mem ? v[q].x * (uint32_t)sizeof(T) : kOob;
+                    p.mem_zb[task] = mem ? (v[q].y - zm_sector) * (uint32_t)sizeof(T) : kOob;
+                }
+            }
         }
         if (threadIdx.x < kCtrPad) ((T *)p.ctr)[st.selems + threadIdx.x] = (T)0;
     } else if (kByY) { // built on the host (st.mtab), contiguous for the group's stars
@@ -1298,23 +1309,24 @@ __global__ __launch_bounds__(kBlock) void k_star_main(dk_tables t, dk_stars st,
         return;
     }
     // stage the slabs as FP64, the values of a set side by side (element e:
-    // u = e % spl, then bi, then set; at r * str + u); four loads in flight
-    for (int e0 = threadIdx.x; e0 < nel; e0 += 4 * kBlock) {
-        double v[4];
+    // u = e % spl, then bi, then set; at r * str + u); eight loads in flight,
+    // issued without branches (elements past nel load the group's first
+    // value and are not stored)
+    constexpr int kSL = 8;
+    for (int e0 = threadIdx.x; e0 < nel; e0 += kSL * kBlock) {
+        T v[kSL];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < kSL; ++q) {
             const int e = e0 + q * kBlock;
-            v[q] = 0.0;
-            if (e < nel) {
-                const int u = e & (spl - 1), r = e >> lsp, set = r / S, bi = r - set * S;
-                const uint32_t gi = gstart[set * spl + u] + (uint32_t)bi;
-                if (ok(gi, n)) v[q] = (double)c[gi];
-            }
+            const int ec = e < nel ? e : 0;
+            const int u = ec & (spl - 1), r = ec >> lsp, set = r / S, bi = r - set * S;
+            const uint32_t gi = gstart[set * spl + u] + (uint32_t)bi;
+            v[q] = ok(gi, n) ? c[gi] : (T)0;
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
+        for (int q = 0; q < kSL; ++q) {
             const int e = e0 + q * kBlock;
-            if (e < nel) own[(e >> lsp) * str + (e & (spl - 1))] = v[q];
+            if (e < nel) own[(e >> lsp) * str + (e & (spl - 1))] = (double)v[q];
         }
     }
     __syncthreads();
