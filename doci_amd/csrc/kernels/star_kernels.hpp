@@ -608,9 +608,11 @@ __device__ __forceinline__ int opaque(int v, int tie)
 // loaded while the last chunk's FMAs run.  The scheduling barriers keep one
 // chunk in flight beside the one in use.
 constexpr int kKC = 8;
+// skip: a row whose K entry is a structural zero (the diagonal x = y of an
+// A column), left out when it is a constant after unrolling; -1 for none.
 template <int NR, int NOUT>
 __device__ __forceinline__ void col_fma(const double *k, const double *kn, double v, double (&out)[NOUT],
-                                        double (&kbuf)[kKC])
+                                        double (&kbuf)[kKC], int skip = -1)
 {
     constexpr int NCH = (NR + kKC - 1) / kKC;
 #pragma unroll
@@ -624,7 +626,8 @@ __device__ __forceinline__ void col_fma(const double *k, const double *kn, doubl
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int i = 0; i < kKC; ++i)
-            if (kKC * ch + i < NR) out[kKC * ch + i] = __builtin_fma(kbuf[i], v, out[kKC * ch + i]);
+            if (kKC * ch + i < NR && kKC * ch + i != skip)
+                out[kKC * ch + i] = __builtin_fma(kbuf[i], v, out[kKC * ch + i]);
         __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
         for (int i = 0; i < kKC; ++i) kbuf[i] = knx[i];
@@ -960,7 +963,7 @@ __global__ __launch_bounds__(kBlock, DK_COLB_WAVES) void k_star_colb(dk_tables t
             if ((ym >> y) & 1) {
                 const uint64_t rest = ym & ~((2ull << y) - 1);
                 const double *kn = rest ? kp + (__ffsll((unsigned long long)rest) - 1) * R : kctr;
-                col_fma<R>(kp + y * R, kn, (double)v[y], out, kbuf);
+                col_fma<R>(kp + y * R, kn, (double)v[y], out, kbuf, y);
             }
         }
         if (has_ctr) {
