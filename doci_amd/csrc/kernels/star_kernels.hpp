@@ -129,7 +129,7 @@ template <typename T>
 __host__ __device__ size_t main_lds(uint32_t L, uint32_t N, const dk_stars &st)
 {
     return a8(sizeof(double) * (st.b * st.b + 1)) + a8(sizeof(uint32_t) * (st.b + 1) * (N + 2)) +
-           a8(sizeof(double) * st.owns) + 2 * a8(sizeof(uint32_t) * st.gslabs);
+           a8(sizeof(T) * st.owns) + 2 * a8(sizeof(uint32_t) * st.gslabs);
 }
 
 // One lane per (star, member offset beta).  The lane loads the member slabs'
@@ -1168,7 +1168,7 @@ __host__ __device__ constexpr int main_spl(int nsl) { return (nsl & 3) == 0 ? 4 
 template <int SPL, typename T, bool kChecked, typename OK>
 __device__ __forceinline__ void main_sets(const double *__restrict__ D, double shift,
                                           const T *__restrict__ Z, T *__restrict__ sigma,
-                                          const double *own, const uint32_t *gstart,
+                                          const T *own, const uint32_t *gstart,
                                           const uint32_t *gzb, const char *Kbb,
                                           __amdgpu_buffer_rsrc_t hrsrc, int S, int nsets, int ns,
                                           int nhp, uint32_t n, uint32_t ztot, uint32_t kbytes, OK &&ok)
@@ -1178,7 +1178,7 @@ __device__ __forceinline__ void main_sets(const double *__restrict__ D, double s
     for (int w = threadIdx.x; w < nsets * S; w += kBlock) {
         const int set = w / S, bi = w - set * S;
         constexpr int STR = main_stride(SPL);
-        const double *ob = own + (size_t)set * S * STR;
+        const T *ob = own + (size_t)set * S * STR;
         // the first hop-table rows, then the slots and the diagonal (HBM),
         // all in flight together; each later chunk of rows is loaded while
         // the one before is used
@@ -1225,10 +1225,10 @@ __device__ __forceinline__ void main_sets(const double *__restrict__ D, double s
                     // J, and the byte offset of K in Kbb (which starts the LDS)
                     const uint32_t J = hv[i0 + i] & 0xffffu, kb = hv[i0 + i] >> 16;
                     const bool g = ok(J, (uint32_t)S) && ok(kb, kbytes);
-                    const double *o = (const double *)((const char *)ob + (g ? J : 0u) * (8u * STR));
+                    const T *o = (const T *)((const char *)ob + (g ? J : 0u) * ((uint32_t)sizeof(T) * STR));
                     Kv[i] = g ? *(const double *)(Kbb + kb) : 0.0;
 #pragma unroll
-                    for (int u = 0; u < SPL; ++u) ov[i][u] = o[u];
+                    for (int u = 0; u < SPL; ++u) ov[i][u] = (double)o[u];
                 }
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
@@ -1250,7 +1250,7 @@ __device__ __forceinline__ void main_sets(const double *__restrict__ D, double s
                 const uint32_t zi = zb + (uint32_t)(q * S);
                 if (ok(zi, ztot)) zs += (double)Z[zi];
             }
-            if (ok(I[u], n)) sigma[I[u]] = (T)(((dI[u] - shift) * ob[bi * STR + u] + acc[u]) + zs);
+            if (ok(I[u], n)) sigma[I[u]] = (T)(((dI[u] - shift) * (double)ob[bi * STR + u] + acc[u]) + zs);
         }
     }
 }
@@ -1282,9 +1282,9 @@ __global__ __launch_bounds__(kBlock) void k_star_main(dk_tables t, dk_stars st,
     };
     double *Kbb = lds_raw; // Kbb[o * b + t] = K between B orbitals o and t; Kbb[b * b] = 0
     uint32_t *bt = (uint32_t *)((char *)lds_raw + a8(sizeof(double) * (b * b + 1)));
-    double *own = (double *)((char *)bt + a8(sizeof(uint32_t) * (b + 1) * (N + 2)));
+    T *own = (T *)((char *)bt + a8(sizeof(uint32_t) * (b + 1) * (N + 2)));
     // the group's slab starts and slot bases
-    uint32_t *gstart = (uint32_t *)((char *)own + a8(sizeof(double) * st.owns));
+    uint32_t *gstart = (uint32_t *)((char *)own + a8(sizeof(T) * st.owns));
     uint32_t *gzb = gstart + a8(sizeof(uint32_t) * st.gslabs) / sizeof(uint32_t);
     for (int i = threadIdx.x; i <= b * b; i += kBlock) {
         const int o = i / b, tt = i - o * b;
@@ -1329,7 +1329,7 @@ __global__ __launch_bounds__(kBlock) void k_star_main(dk_tables t, dk_stars st,
 #pragma unroll
         for (int q = 0; q < kSL; ++q) {
             const int e = e0 + q * kBlock;
-            if (e < nel) own[(e >> lsp) * str + (e & (spl - 1))] = (double)v[q];
+            if (e < nel) own[(e >> lsp) * str + (e & (spl - 1))] = v[q];
         }
     }
     __syncthreads();
