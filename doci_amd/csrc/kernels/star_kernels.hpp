@@ -1219,7 +1219,8 @@ __device__ __forceinline__ void main_sets(const double *__restrict__ D, double s
             // four hops at a time: every LDS read issued before the products
 #pragma unroll
             for (int i0 = 0; i0 < kHB; i0 += 4) {
-                double Kv[4], ov[4][SPL];
+                double Kv[4];
+                T ov[4][SPL];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     // J, and the byte offset of K in Kbb (which starts the LDS)
@@ -1228,13 +1229,13 @@ __device__ __forceinline__ void main_sets(const double *__restrict__ D, double s
                     const T *o = (const T *)((const char *)ob + (g ? J : 0u) * ((uint32_t)sizeof(T) * STR));
                     Kv[i] = g ? *(const double *)(Kbb + kb) : 0.0;
 #pragma unroll
-                    for (int u = 0; u < SPL; ++u) ov[i][u] = (double)o[u];
+                    for (int u = 0; u < SPL; ++u) ov[i][u] = o[u];
                 }
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int i = 0; i < 4; ++i)
 #pragma unroll
-                    for (int u = 0; u < SPL; ++u) acc[u] += Kv[i] * ov[i][u];
+                    for (int u = 0; u < SPL; ++u) acc[u] += Kv[i] * (double)ov[i][u];
             }
 #pragma unroll
             for (int i = 0; i < kHB; ++i) hv[i] = hn[i];
