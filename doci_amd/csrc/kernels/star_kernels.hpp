@@ -37,9 +37,6 @@ constexpr int kCtrPad = 64;
 // a slab (< 2^16 elements of at most 8 bytes) added to kOob does not wrap.
 constexpr uint32_t kOob = 0xfff00000u;
 constexpr uint64_t kBufLimit = 0xffe00000ull;
-#ifndef DK_COLB_WAVES
-#define DK_COLB_WAVES 3
-#endif
 
 __host__ __device__ __forceinline__ size_t a8(size_t x) { return (x + 7) & ~(size_t)7; }
 
@@ -853,7 +850,7 @@ __device__ __forceinline__ void buf_store(__amdgpu_buffer_rsrc_t r, uint32_t off
 // * the orbitals in every lane's star: the AND of the A-parts of the stars
 //   the wave's columns touch, a scalar loop over the first and last lane's.
 template <typename T, bool kChecked, int AMAX, int BMAX, bool kExact>
-__global__ __launch_bounds__(kBlock, DK_COLB_WAVES) void k_star_colb(dk_tables t, dk_stars st, const T *__restrict__ c,
+__global__ __launch_bounds__(kBlock, 3) void k_star_colb(dk_tables t, dk_stars st, const T *__restrict__ c,
                                                          T *__restrict__ Z, uint32_t *bad,
                                                          const dk_sigma_io *io,
                                                          const double *__restrict__ KP)
