@@ -176,7 +176,7 @@ typedef struct {             /* constant per instance */
 } dk_binst;
 
 enum { DK_B_IDLE = 0, DK_B_ACTIVE = 1, DK_B_CONVERGED = 2, DK_B_MAXITS = 3, DK_B_BREAKDOWN = 4 };
-enum { DK_BF_FIRST = 1, DK_BF_ORTHO2 = 2, DK_BF_DEFLATE = 4 };
+enum { DK_BF_FIRST = 1, DK_BF_ORTHO = 2, DK_BF_DEFLATE = 4 };
 
 typedef struct {             /* summary copied to the host every iteration */
     int32_t status, m, its, flags;
@@ -191,13 +191,19 @@ typedef struct {             /* mutable per instance */
     int32_t vslot[DK_BMAX_VEC], wslot[DK_BMAX_VEC];
     int32_t dv[DK_BKEEP_MAX], dw[DK_BKEEP_MAX]; /* deflation targets */
     uint64_t vused, wused;   /* slot occupancy bitmasks */
-    double tol, otol, shift, theta;  /* theta: lowest eigenvalue of V^T (H - shift) V */
-    double s[DK_BMAX_VEC];   /* V_j = s_j * stored V_j (same for W_j) */
-    double g[DK_BMAX_VEC];   /* ortho coefficients on stored V */
-    double y[DK_BMAX_VEC];   /* residual coefficients on stored V / W */
+    double tol, otol, shift, theta;  /* theta: lowest eigenvalue of the projected H - shift;
+                                        otol: a correction whose part orthogonal to the
+                                        basis is below otol of its norm squared is
+                                        projected out explicitly */
+    double s[DK_BMAX_VEC];   /* T_j = s_j * stored T_j (same for W_j), |T_j| = 1 */
+    double g[DK_BMAX_VEC];   /* projection coefficients on stored T (explicit projection) */
+    double y[DK_BMAX_VEC];   /* residual coefficients on stored T / W */
     double defl[DK_BKEEP_MAX][DK_BMAX_VEC];
-    double A[DK_BMAX_VEC * DK_BMAX_VEC]; /* projected matrix in the current eigenbasis */
-    double Y[DK_BMAX_VEC * DK_BMAX_VEC]; /* eigenvectors, column k = Y[j*DK_BMAX_VEC+k] */
+    /* The basis T (raw corrections) = Q R with Q orthonormal (never formed):
+     * R upper triangular, R[i*DK_BMAX_VEC+j] for i <= j */
+    double R[DK_BMAX_VEC * DK_BMAX_VEC];
+    double A[DK_BMAX_VEC * DK_BMAX_VEC]; /* projected matrix Q^T (H - shift) Q in its eigenbasis */
+    double Y[DK_BMAX_VEC * DK_BMAX_VEC]; /* eigenvectors in Q, column k = Y[j*DK_BMAX_VEC+k] */
     dk_sigma_io sio;         /* star-kernel sigma operands of this iteration */
 } dk_bstate;
 

@@ -23,6 +23,11 @@
 
 #include "internal.h"
 
+/* a correction whose part orthogonal to an instance's basis is below this
+ * fraction of its norm squared is projected out explicitly (an ORTHO pass);
+ * any other joins the basis as it is */
+#define DK_BATCH_DEP_TOL 0.05
+
 typedef struct {
     int state;              /* 0 queued, 1 running, 2 finished */
     int want_mixed;
@@ -631,7 +636,7 @@ static int admit(doci_batch *b, int ri, int slot)
     ad->t = bi->slots;
     ad->state = b->d_state + slot;
     ad->tol = r->prec ? b->o.switch_tol : b->o.tol;
-    ad->otol = r->prec ? 1e-6 : 1e-12;
+    ad->otol = DK_BATCH_DEP_TOL;
     b->pend_tiles += ad->ntiles;
     if (L > b->pend_maxL) b->pend_maxL = L;
     if (N > b->pend_maxN) b->pend_maxN = N;
@@ -857,8 +862,6 @@ static int enqueue_iteration(doci_batch *b, int p, int M)
     DKCHK(dk_batch_small(DK_BPH_A0, g->d_list, g->n, b->d_inst, b->d_state, g->d_part, s));
     DKCHK(dk_batch_stream(DK_BOP_ORTHO, p, M, b->d_inst, b->d_state, g->d_tiles, g->ntile, g->d_part, s));
     DKCHK(dk_batch_small(DK_BPH_B, g->d_list, g->n, b->d_inst, b->d_state, g->d_part, s));
-    DKCHK(dk_batch_stream(DK_BOP_ORTHO2, p, M, b->d_inst, b->d_state, g->d_tiles, g->ntile, g->d_part, s));
-    DKCHK(dk_batch_small(DK_BPH_C, g->d_list, g->n, b->d_inst, b->d_state, g->d_part, s));
     if (g->maxL)
         DKCHK(dk_batch_sigma_lane(p, g->maxL, g->maxN, b->d_inst, b->d_state, g->d_tiles, g->ntile, s));
     for (int k = 0; k < g->n; ++k) { /* large instances: star kernel, operands read on the device */
@@ -1048,7 +1051,7 @@ static int restart_fp64(doci_batch *b, brec *r, const dk_bstat *st)
     q->keep = b->o.keep;
     q->max_vec = b->o.max_vec;
     q->tol = b->o.tol;
-    q->otol = 1e-12;
+    q->otol = DK_BATCH_DEP_TOL;
     b->rst_tiles += (r->n + DK_ADMIT_ROWS - 1) / DK_ADMIT_ROWS;
     b->dirty = 1;
     return DOCI_OK;

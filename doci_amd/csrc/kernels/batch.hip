@@ -7,21 +7,30 @@
 // instance's state:
 //
 //   small A0  (instances just started) loop test with the initial residual
-//   ORTHO     t -= sum g_j V_j  (+ norms and overlaps for the check)
-//   small B   loss of orthogonality -> second pass, or scale of V_m
-//   ORTHO2    second classical Gram-Schmidt pass (flagged instances only)
-//   small C   scale of V_m after the second pass
-//   sigma     W_m = (H - shift) V_m
-//   SCOL      new column V_j^T W_m of the projected matrix
+//   ORTHO     flagged instances only: t -= sum g_j T_j (explicit projection
+//             of a nearly dependent correction) or |t| of a first t, with
+//             the norm and overlaps
+//   small B   after ORTHO: t joins the basis (its scale and column of R)
+//   sigma     W_m = (H - shift) T_m
+//   SCOL      new column T_j^T W_m of the projected matrix
 //   small D   bordered update of the projected eigenproblem (cyclic Jacobi,
 //             parallel ordering), lowest Ritz pair; deflation plan when the
 //             basis reaches max_vec (src/davidson.c:83-89)
-//   DEFLATE   V'_k = sum_j Y_jk V_j, W'_k likewise (flagged instances only)
+//   DEFLATE   T'_k = sum_j (R^-1 Y)_jk T_j, W'_k likewise (flagged instances only)
 //   small E   basis = the kept Ritz vectors
-//   RESID     x = V y, r = W y - theta x, t = r / |D - E| with the 1e-12
+//   RESID     x = T y, r = W y - theta x, t = r / |D - E| with the 1e-12
 //             floor (src/davidson.c:214-227), x written in FP64
-//   small A   residual norm, loop test `res > tol && its++ < max_its`,
-//             Gram-Schmidt coefficients of t for the next iteration
+//   small A   residual norm, loop test `res > tol && its++ < max_its`;
+//             the new correction t joins the basis T as it is (its
+//             overlaps with the basis from RESID give the new column of R)
+//             unless it is nearly dependent on it, which flags ORTHO
+//
+// The basis T holds the raw corrections (src/davidson.c:133-176
+// orthogonalises each one; the same subspace here), factored T = Q R with
+// Q orthonormal and never formed: R comes from the overlaps RESID already
+// computes, and the projected matrix is kept in Q, bordered each iteration
+// with the new column of T^T (H - shift) T transformed by R.  This spares
+// the Gram-Schmidt pass over the basis in every iteration.
 //
 // Every partial sum is written and reduced within one iteration's launch
 // sequence, so the tile layout may change between iterations (instances
@@ -250,7 +259,8 @@ __global__ __launch_bounds__(kBlock) void k_bstream(const dk_binst *__restrict__
     const dk_bstate &st = states[ii];
     if (st.st.status != DK_B_ACTIVE || ti >= in.ntiles) return; // ti = 0xffffffff: padding
     const int flags = st.st.flags;
-    if (OP == DK_BOP_ORTHO2 && !(flags & DK_BF_ORTHO2)) return;
+    // explicit projection (and the norm of a first t): flagged instances only
+    if ((OP == DK_BOP_ORTHO || OP == DK_BOP_ORTHO2) && !(flags & (DK_BF_ORTHO | DK_BF_FIRST))) return;
     if (OP == DK_BOP_DEFLATE && !(flags & DK_BF_DEFLATE)) return;
     const int m = st.st.m;
     const uint32_t r0 = ti * in.tile_rows;
@@ -500,12 +510,54 @@ __device__ void jacobi_par(double *A, double *Y, int n)
     }
 }
 
+// The overlaps of a new direction t with the orthonormal Q of the basis
+// (T = Q R): from d_j = <stored T_j, t> and tt = <t, t>, r = R^-T b with
+// b_j = s_j d_j / |t|; returns rho^2 = 1 - |r|^2, the part of t/|t| outside
+// the basis.  One thread.
+__device__ double bs_overlaps(const dk_bstate &st, int m, double tt, const double *d, double *r)
+{
+    const double it = 1 / sqrt(tt);
+    double nr = 0;
+    for (int j = 0; j < m; ++j) {
+        double v = st.s[j] * d[j] * it;
+        for (int i = 0; i < j; ++i) v -= st.R[i * LDY + j] * r[i];
+        r[j] = v / st.R[j * LDY + j];
+        nr += r[j] * r[j];
+    }
+    return 1 - nr;
+}
+
+// t becomes basis vector m with scale 1/|t| and column m of R (r, rho);
+// W_m goes to a free W slot, and the star-kernel sigma operands are set.
+// One thread.
+__device__ void bs_finalize(dk_bstate &st, const dk_binst &in, int m, double tt, const double *r,
+                            double rho)
+{
+    st.s[m] = 1 / sqrt(tt);
+    for (int j = 0; j < m; ++j) st.R[j * LDY + m] = r[j];
+    st.R[m * LDY + m] = rho;
+    st.vslot[m] = st.tslot;
+    const int w = take_slot(st.wused, st.P, st.P);
+    st.wslot[m] = w;
+    st.sin = st.tslot;
+    st.sout = w;
+    if (w < 0) {
+        st.st.status = DK_B_BREAKDOWN;
+        return;
+    }
+    // operands of the star-kernel sigma (instances without lane sigma)
+    st.sio.c = in.slots + (uint64_t)st.tslot * in.slot_stride;
+    st.sio.sigma = in.slots + (uint64_t)w * in.slot_stride;
+    st.sio.shift = st.shift;
+    st.sio.active = 1;
+}
+
 __global__ __launch_bounds__(kSmall) void k_bsmall(int phase, const uint32_t *__restrict__ ilist,
                                                    const dk_binst *__restrict__ insts,
                                                    dk_bstate *__restrict__ states,
                                                    const double *__restrict__ part)
 {
-    __shared__ double red[DK_BPV];
+    __shared__ double red[DK_BPV], rbuf[DK_BMAX_VEC];
     __shared__ double A[DK_BMAX_VEC * LDY], Y[DK_BMAX_VEC * LDY];
     __shared__ int ctl;
     const uint32_t ii = ilist[blockIdx.x];
@@ -535,75 +587,106 @@ __global__ __launch_bounds__(kSmall) void k_bsmall(int phase, const uint32_t *__
                 st.st.status = DK_B_BREAKDOWN;
             else
                 ctl = 1;
-            st.st.flags = flags & ~DK_BF_FIRST;
-        }
-        __syncthreads();
-        if (ctl && !first)
-            for (int j = threadIdx.x; j < m; j += kSmall) st.g[j] = st.s[j] * st.s[j] * red[2 + j];
-        return;
-    }
-    if (phase == DK_BPH_B || phase == DK_BPH_C) {
-        if (phase == DK_BPH_C && !(flags & DK_BF_ORTHO2)) return;
-        reduce_tiles(in, part, m + 1, red);
-        if (threadIdx.x == 0) {
-            const double tn = sqrt(red[0]);
-            double loss = 0;
-            for (int j = 0; j < m; ++j) loss = fmax(loss, fabs(st.s[j] * red[1 + j]));
-            ctl = 0;
-            if (!(tn > 0) || !isfinite(tn)) {
-                st.st.status = DK_B_BREAKDOWN;
-            } else if (phase == DK_BPH_B && m > 0 && loss > st.otol * tn) {
-                st.st.flags = flags | DK_BF_ORTHO2;
-                ctl = 1; // second pass coefficients below
-            } else {
-                // t becomes V_m with scale 1/|t|; W_m goes to a free W slot
-                st.s[m] = 1 / tn;
-                st.vslot[m] = st.tslot;
-                const int w = take_slot(st.wused, st.P, st.P);
-                st.wslot[m] = w;
-                st.sin = st.tslot;
-                st.sout = w;
-                st.st.flags = flags & ~DK_BF_ORTHO2;
-                if (w < 0) {
-                    st.st.status = DK_B_BREAKDOWN;
+            int nf = flags & ~DK_BF_FIRST;
+            if (ctl && first) {
+                nf |= DK_BF_ORTHO; // the ORTHO pass measures |t| (m = 0: no projection)
+            } else if (ctl) {
+                // t joins the basis as it is unless it is nearly dependent on it
+                double *r = rbuf;
+                const double tt = red[1];
+                const double rho2 = bs_overlaps(st, m, tt, red + 2, r);
+                if (rho2 > st.otol) {
+                    bs_finalize(st, in, m, tt, r, sqrt(rho2));
                 } else {
-                    // operands of the star-kernel sigma (instances without lane sigma)
-                    st.sio.c = in.slots + (uint64_t)st.tslot * in.slot_stride;
-                    st.sio.sigma = in.slots + (uint64_t)w * in.slot_stride;
-                    st.sio.shift = st.shift;
-                    st.sio.active = 1;
+                    // explicit projection t -= T c, c = R^-1 R^-T <T, t>
+                    const double nt = sqrt(tt);
+                    for (int j = m - 1; j >= 0; --j) {
+                        double v = r[j] * nt;
+                        for (int i = j + 1; i < m; ++i) v -= st.R[j * LDY + i] * st.g[i];
+                        st.g[j] = v / st.R[j * LDY + j];
+                    }
+                    for (int j = 0; j < m; ++j) st.g[j] *= st.s[j];
+                    nf |= DK_BF_ORTHO;
                 }
             }
+            st.st.flags = nf;
         }
-        __syncthreads();
-        if (ctl)
-            for (int j = threadIdx.x; j < m; j += kSmall) st.g[j] = st.s[j] * st.s[j] * red[1 + j];
+        return;
+    }
+    if (phase == DK_BPH_B) {
+        // after an explicit projection (or the first t's norm): the
+        // projected t joins the basis
+        if (!(flags & DK_BF_ORTHO)) return;
+        reduce_tiles(in, part, m + 1, red);
+        if (threadIdx.x == 0) {
+            double *r = rbuf;
+            const double tt = red[0];
+            const double rho2 = tt > 0 && isfinite(tt) ? bs_overlaps(st, m, tt, red + 1, r) : -1.0;
+            if (!(rho2 > 0))
+                st.st.status = DK_B_BREAKDOWN;
+            else
+                bs_finalize(st, in, m, tt, r, sqrt(rho2));
+            st.st.flags = flags & ~DK_BF_ORTHO;
+        }
         return;
     }
     if (phase == DK_BPH_D) {
         if (threadIdx.x == 0) st.sio.active = 0; // this iteration's sigma has run
         reduce_tiles(in, part, m + 1, red);
         const int mn = m + 1;
-        // column m of the projected matrix in true (normalised) vectors
-        __shared__ double col[DK_BMAX_VEC];
+        // column m of T^T (H - shift) T (T_j = s_j * stored T_j)
+        __shared__ double col[DK_BMAX_VEC], z[DK_BMAX_VEC], u[DK_BMAX_VEC], w[DK_BMAX_VEC],
+            v[DK_BMAX_VEC], rr[DK_BMAX_VEC];
         for (int j = threadIdx.x; j <= m; j += kSmall) col[j] = st.s[j] * st.s[m] * red[j];
         for (int e = threadIdx.x; e < m * m; e += kSmall) {
             const int i = e / m, j = e - i * m;
             A[i * LDY + j] = st.A[i * LDY + j];
             Y[i * LDY + j] = st.Y[i * LDY + j];
         }
+        for (int j = threadIdx.x; j < m; j += kSmall) rr[j] = st.R[j * LDY + m];
         __syncthreads();
-        // border the rotated matrix: b = Y^T col, A' = [A b; b^T col_m]
+        // in Q (T = Q R, R = [R0 r; 0 rho]): column m of Q^T (H - shift) Q is
+        //   (R0^-T c - Ahat r) / rho  and  (c_m - 2 r.z + r.Ahat r) / rho^2,
+        // z = R0^-T c, Ahat = Y A Y^T the previous projected matrix
+        if (threadIdx.x == 0)
+            for (int j = 0; j < m; ++j) {
+                double a = col[j];
+                for (int i = 0; i < j; ++i) a -= st.R[i * LDY + j] * z[i];
+                z[j] = a / st.R[j * LDY + j];
+            }
+        __syncthreads();
+        for (int k = threadIdx.x; k < m; k += kSmall) { // w = Y^T z, u = Y^T r
+            double a = 0, b = 0;
+            for (int j = 0; j < m; ++j) {
+                a += Y[j * LDY + k] * z[j];
+                b += Y[j * LDY + k] * rr[j];
+            }
+            w[k] = a;
+            u[k] = b;
+        }
+        __syncthreads();
+        for (int k = threadIdx.x; k < m; k += kSmall) { // v = A u
+            double a = 0;
+            for (int j = 0; j < m; ++j) a += A[k * LDY + j] * u[j];
+            v[k] = a;
+        }
+        __syncthreads();
+        // border the rotated matrix: A' = [A b; b^T corner], b = Y^T (column in Q)
+        const double rho = st.R[m * LDY + m];
         for (int k = threadIdx.x; k < m; k += kSmall) {
-            double b = 0;
-            for (int j = 0; j < m; ++j) b += Y[j * LDY + k] * col[j];
+            const double b = (w[k] - v[k]) / rho;
             A[k * LDY + m] = b;
             A[m * LDY + k] = b;
             Y[k * LDY + m] = 0;
             Y[m * LDY + k] = 0;
         }
         if (threadIdx.x == 0) {
-            A[m * LDY + m] = col[m];
+            double rz = 0, uv = 0;
+            for (int j = 0; j < m; ++j) {
+                rz += rr[j] * z[j];
+                uv += u[j] * v[j];
+            }
+            A[m * LDY + m] = (col[m] - 2 * rz + uv) / (rho * rho);
             Y[m * LDY + m] = 1;
         }
         __syncthreads();
@@ -642,16 +725,21 @@ __global__ __launch_bounds__(kSmall) void k_bsmall(int phase, const uint32_t *__
             st.A[i * LDY + j] = A[i * LDY + j];
             st.Y[i * LDY + j] = Y[i * LDY + j];
         }
-        if (ctl) {
-            for (int e = threadIdx.x; e < st.keep * mn; e += kSmall) {
-                const int k = e / mn, j = e - k * mn;
-                st.defl[k][j] = Y[j * LDY + order[k]] * st.s[j];
+        // Ritz vectors on the stored vectors: R^-1 (Y column), times the scales
+        const int nvec = ctl ? st.keep : 1;
+        if (threadIdx.x < nvec) {
+            const int k = threadIdx.x;
+            double *out = ctl ? st.defl[k] : st.y;
+            for (int j = mn - 1; j >= 0; --j) {
+                double a = Y[j * LDY + order[k]];
+                for (int i = j + 1; i < mn; ++i) a -= st.R[j * LDY + i] * out[i];
+                out[j] = a / st.R[j * LDY + j];
             }
-            // the kept Ritz values, for small E
-            for (int k = threadIdx.x; k < st.keep; k += kSmall) st.g[k] = A[order[k] * LDY + order[k]];
-        } else {
-            for (int j = threadIdx.x; j < mn; j += kSmall) st.y[j] = Y[j * LDY + order[0]] * st.s[j];
+            for (int j = 0; j < mn; ++j) out[j] *= st.s[j];
         }
+        // the kept Ritz values, for small E
+        if (ctl)
+            for (int k = threadIdx.x; k < st.keep; k += kSmall) st.g[k] = A[order[k] * LDY + order[k]];
         return;
     }
     if (phase == DK_BPH_E) {
@@ -672,6 +760,7 @@ __global__ __launch_bounds__(kSmall) void k_bsmall(int phase, const uint32_t *__
                 for (int j = 0; j < keep; ++j) {
                     st.A[i * LDY + j] = i == j ? st.g[i] : 0.0;
                     st.Y[i * LDY + j] = i == j ? 1.0 : 0.0;
+                    st.R[i * LDY + j] = i == j ? 1.0 : 0.0; // the kept Ritz vectors are orthonormal
                 }
             st.theta = st.g[0];
             st.st.energy = st.theta + st.shift;
