@@ -7,13 +7,19 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include <type_traits>
+
 #include "../dk.h"
 
 namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
-constexpr int kMaxBlocks = 2048;
+// three workgroups per CU of the 256: streams of several vectors reach a
+// higher HBM rate with these twelve waves per CU than with more of them in
+// flight (measured at 2^27 elements: FP64 multidot of 8 vectors 1.55 ms at
+// 768 workgroups, 1.58 at 2048; residual with W dots 3.9 vs 4.3 ms)
+constexpr int kMaxBlocks = 768;
 // the residual's 2 + 2m values fit the reduction buffers the host sizes for
 // DK_MAX_VEC + 8 values
 static_assert(2 + 2 * DK_RESIDUAL_WDOTS_MAX <= DK_MAX_VEC + 8, "residual dots");
@@ -151,83 +157,139 @@ constexpr int vw_for()
     return M <= 16 ? w : (M <= 32 ? (w >= 2 ? w / 2 : 1) : (w >= 4 ? w / 4 : 1));
 }
 
+// The M-vector kernels below run at a compile-time M (a bucket >= m): the
+// host fills the vector slots j >= m with the last real vector (or y) and
+// their coefficients with 0, so every load and FMA is unconditional and all
+// M loads of a step are issued before the first product (a load under a
+// `j < m` branch waits for the one before it); the repeated slots read lines
+// the same lane has just loaded, and their sums are never stored.
+//
+// Addressing: the vectors are walked in windows of kWin lane steps; per
+// window the base addresses advance once (scalar), and inside it every load
+// is base + a 32-bit byte offset shared by all vectors, which the hardware
+// takes as an SGPR base plus one VGPR offset (no 64-bit address per vector).
+
+// lane steps per window: 16-byte groups, the FP64 diagonal of an FP32 step
+// twice that; 2^26 steps keep every offset below 2^31
+constexpr int64_t kWin = int64_t(1) << 26;
+
+template <typename VT>
+__device__ __forceinline__ VT ldb(const void *p, uint32_t off)
+{
+    return *(const VT *)((const char *)p + off);
+}
+template <typename VT>
+__device__ __forceinline__ void stb(void *p, uint32_t off, const VT &v)
+{
+    *(VT *)((char *)p + off) = v;
+}
+template <int M>
+__device__ __forceinline__ void advance(Ptrs &X, size_t bytes)
+{
+#pragma unroll
+    for (int j = 0; j < M; ++j) X.p[j] = (const char *)X.p[j] + bytes;
+}
+// an offset the compiler cannot prove equal to its input (a second load of
+// the same element is issued again instead of being kept in registers)
+__device__ __forceinline__ uint32_t launder(uint32_t v)
+{
+    asm volatile("" : "+v"(v));
+    return v;
+}
+
+// Runs step(off, VW-tag) over all n elements: windows of VW-element groups
+// at byte offsets off from the current bases, adv(bytes) moving the bases
+// (one set of them, carried from window to window) past each window, then
+// the n % VW tail one element per lane.
+template <typename T, int VW, typename Step, typename Adv>
+__device__ __forceinline__ void stream(int64_t n, Step &&step, Adv &&adv)
+{
+    typedef typename Vec<T, VW>::type VT;
+    const int64_t nq = n / VW;
+    const uint32_t stride = gridDim.x * kBlock, tid = blockIdx.x * kBlock + threadIdx.x;
+    for (int64_t w0 = 0; w0 < nq; w0 += kWin) {
+        const uint32_t wn = (uint32_t)(nq - w0 < kWin ? nq - w0 : kWin);
+        for (uint32_t q = tid; q < wn; q += stride)
+            step(q * (uint32_t)sizeof(VT), std::integral_constant<int, VW>());
+        adv((size_t)wn * sizeof(VT));
+    }
+    const uint32_t nt = (uint32_t)(n - nq * VW);
+    for (uint32_t i = tid; i < nt; i += stride)
+        step(i * (uint32_t)sizeof(T), std::integral_constant<int, 1>());
+}
+
+// vectors loaded together before their products: all of them up to 16,
+// groups of 8 beyond (the registers of more would cost occupancy)
+template <int M>
+constexpr int kChunk = M <= 16 ? M : 8;
+static_assert(24 % kChunk<24> == 0 && 32 % kChunk<32> == 0 && 64 % kChunk<64> == 0, "whole groups");
+
 // ---- multidot: out[0] = <y,y>, out[1+j] = <X_j, y> -------------------------
 template <typename T, int M>
 __global__ __launch_bounds__(kBlock) void k_multidot(Ptrs X, int m, const T *__restrict__ y,
                                                      int64_t n, double *__restrict__ red)
 {
-    constexpr int VW = 16 / (int)sizeof(T);
+    constexpr int C = kChunk<M>;
     double acc[M + 1];
 #pragma unroll
     for (int k = 0; k <= M; ++k) acc[k] = 0;
-    const int64_t stride = (int64_t)gridDim.x * kBlock, nq = n / VW;
-    const int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    for (int64_t q = tid; q < nq; q += stride) {
-        const auto yv = ldv<T, VW>(y, q);
+    const char *yb = (const char *)y;
+    stream<T, vw_for<T, M>()>(n, [&](uint32_t off, auto vwt) {
+        constexpr int VW = decltype(vwt)::value;
+        typedef typename Vec<T, VW>::type VT;
+        const VT yv = ldb<VT>(yb, off);
 #pragma unroll
         for (int e = 0; e < VW; ++e) acc[0] += el<T, VW>(yv, e) * el<T, VW>(yv, e);
 #pragma unroll
-        for (int j = 0; j < M; ++j)
-            if (j < m) {
-                const auto xv = ldv<T, VW>(X.p[j], q);
+        for (int j0 = 0; j0 < M; j0 += C) {
+            VT xv[C];
 #pragma unroll
-                for (int e = 0; e < VW; ++e) acc[1 + j] += el<T, VW>(xv, e) * el<T, VW>(yv, e);
-            }
-    }
-    for (int64_t i = nq * VW + tid; i < n; i += stride) {
-        const double yi = (double)y[i];
-        acc[0] += yi * yi;
+            for (int j = 0; j < C; ++j) xv[j] = ldb<VT>(X.p[j0 + j], off);
 #pragma unroll
-        for (int j = 0; j < M; ++j)
-            if (j < m) acc[1 + j] += ld<T>(X.p[j], i) * yi;
-    }
+            for (int j = 0; j < C; ++j)
+#pragma unroll
+                for (int e = 0; e < VW; ++e)
+                    acc[1 + j0 + j] += el<T, VW>(xv[j], e) * el<T, VW>(yv, e);
+        }
+    }, [&](size_t b) { advance<M>(X, b); yb += b; });
     block_partials<M + 1>(acc, m + 1, red);
 }
 
 // ---- ortho: y -= sum g_j X_j; out[0] = <y,y>, out[1+j] = <X_j,y> -----------
-template <typename T, int M, int VW>
-__device__ __forceinline__ void ortho_step(const Ptrs &X, int m, const Coef &g, T *__restrict__ y,
-                                           int64_t q, double (&acc)[M + 1])
-{
-    typedef typename Vec<T, VW>::type VT;
-    VT xv[M];
-    VT yv = ((VT *)y)[q];
-    double yd[VW];
-#pragma unroll
-    for (int e = 0; e < VW; ++e) yd[e] = el<T, VW>(yv, e);
-#pragma unroll
-    for (int j = 0; j < M; ++j)
-        if (j < m) {
-            xv[j] = ldv<T, VW>(X.p[j], q);
-#pragma unroll
-            for (int e = 0; e < VW; ++e) yd[e] -= g.g[j] * el<T, VW>(xv[j], e);
-        }
-#pragma unroll
-    for (int e = 0; e < VW; ++e) {
-        set_el<T, VW>(yv, e, (T)yd[e]);
-        yd[e] = el<T, VW>(yv, e);
-        acc[0] += yd[e] * yd[e];
-    }
-    ((VT *)y)[q] = yv;
-#pragma unroll
-    for (int j = 0; j < M; ++j)
-        if (j < m)
-#pragma unroll
-            for (int e = 0; e < VW; ++e) acc[1 + j] += el<T, VW>(xv[j], e) * yd[e];
-}
-
 template <typename T, int M>
 __global__ __launch_bounds__(kBlock) void k_ortho(Ptrs X, int m, Coef g, T *__restrict__ y,
                                                   int64_t n, double *__restrict__ red)
 {
-    constexpr int VW = vw_for<T, M>();
     double acc[M + 1];
 #pragma unroll
     for (int k = 0; k <= M; ++k) acc[k] = 0;
-    const int64_t stride = (int64_t)gridDim.x * kBlock, nq = n / VW;
-    const int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    for (int64_t q = tid; q < nq; q += stride) ortho_step<T, M, VW>(X, m, g, y, q, acc);
-    for (int64_t i = nq * VW + tid; i < n; i += stride) ortho_step<T, M, 1>(X, m, g, y, i, acc);
+    char *yw = (char *)y;
+    stream<T, vw_for<T, M>()>(n, [&](uint32_t off, auto vwt) {
+        constexpr int VW = decltype(vwt)::value;
+        typedef typename Vec<T, VW>::type VT;
+        VT xv[M];
+        VT yv = ldb<VT>(yw, off);
+        double yd[VW];
+#pragma unroll
+        for (int e = 0; e < VW; ++e) yd[e] = el<T, VW>(yv, e);
+#pragma unroll
+        for (int j = 0; j < M; ++j) xv[j] = ldb<VT>(X.p[j], off);
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+#pragma unroll
+            for (int e = 0; e < VW; ++e) yd[e] -= g.g[j] * el<T, VW>(xv[j], e);
+#pragma unroll
+        for (int e = 0; e < VW; ++e) {
+            set_el<T, VW>(yv, e, (T)yd[e]);
+            yd[e] = el<T, VW>(yv, e);
+            acc[0] += yd[e] * yd[e];
+        }
+        stb<VT>(yw, off, yv);
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+#pragma unroll
+            for (int e = 0; e < VW; ++e) acc[1 + j] += el<T, VW>(xv[j], e) * yd[e];
+    }, [&](size_t b) { advance<M>(X, b); yw += b; });
     block_partials<M + 1>(acc, m + 1, red);
 }
 
@@ -237,115 +299,122 @@ __global__ __launch_bounds__(kBlock) void k_ortho(Ptrs X, int m, Coef g, T *__re
 // lines are still in L2, and holding the W values in registers instead
 // costs more occupancy than the second loads cost (measured at L = 34: m = 8
 // residual 4.26 ms with the second loads, 4.60 ms with the values held).
-template <typename P>
-__device__ __forceinline__ P launder(P p)
-{
-    asm volatile("" : "+v"(p));
-    return p;
-}
-
-template <typename T, int M, int VW, bool WB>
-__device__ __forceinline__ void residual_step(const Ptrs &V, const Ptrs &W, int m, const Coef &a,
-                                              double theta, double shift,
-                                              const double *__restrict__ D, T *__restrict__ t,
-                                              T *__restrict__ xo, int64_t q,
-                                              double (&acc)[2 + (WB ? 2 : 1) * M])
-{
-    typedef typename Vec<T, VW>::type VT;
-    VT vv[M];
-    double x[VW], w[VW];
-#pragma unroll
-    for (int e = 0; e < VW; ++e) x[e] = w[e] = 0;
-#pragma unroll
-    for (int j = 0; j < M; ++j)
-        if (j < m) {
-            vv[j] = ldv<T, VW>(V.p[j], q);
-            const VT wv = ldv<T, VW>(W.p[j], q);
-#pragma unroll
-            for (int e = 0; e < VW; ++e) {
-                x[e] += a.g[j] * el<T, VW>(vv[j], e);
-                w[e] += a.g[j] * el<T, VW>(wv, e);
-            }
-        }
-    VT tv, xv;
-    double td[VW];
-#pragma unroll
-    for (int e = 0; e < VW; ++e) {
-        const double r = w[e] - (theta - shift) * x[e];
-        const double d = fabs(D[q * VW + e] - theta);
-        const double tval = d > 1e-12 ? r / d : r / 1e-12;
-        set_el<T, VW>(tv, e, (T)tval);
-        set_el<T, VW>(xv, e, (T)x[e]);
-        td[e] = el<T, VW>(tv, e);
-        acc[0] += r * r;
-        acc[1] += td[e] * td[e];
-    }
-    ((VT *)t)[q] = tv;
-    if (xo) ((VT *)xo)[q] = xv;
-#pragma unroll
-    for (int j = 0; j < M; ++j)
-        if (j < m)
-#pragma unroll
-            for (int e = 0; e < VW; ++e) acc[2 + j] += el<T, VW>(vv[j], e) * td[e];
-    if constexpr (WB) {
-#pragma unroll
-        for (int j = 0; j < M; ++j)
-            if (j < m) {
-                const VT wv = ldv<T, VW>(launder(W.p[j]), q);
-#pragma unroll
-                for (int e = 0; e < VW; ++e) acc[2 + M + j] += el<T, VW>(wv, e) * td[e];
-            }
-    }
-}
-
 template <typename T, int M, bool WB>
 __global__ __launch_bounds__(kBlock) void k_residual(Ptrs V, Ptrs W, int m, Coef a, double theta,
                                                      double shift, const double *__restrict__ D,
                                                      T *__restrict__ t, T *__restrict__ xo,
                                                      int64_t n, double *__restrict__ red)
 {
-    constexpr int VW = vw_for<T, M>();
+    constexpr int C = M <= 8 ? M : (M % 8 == 0 ? 8 : M / 2);
+    static_assert(M % C == 0, "whole groups");
     constexpr int NA = 2 + (WB ? 2 : 1) * M;
     double acc[NA];
 #pragma unroll
     for (int k = 0; k < NA; ++k) acc[k] = 0;
-    const int64_t stride = (int64_t)gridDim.x * kBlock, nq = n / VW;
-    const int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    for (int64_t q = tid; q < nq; q += stride)
-        residual_step<T, M, VW, WB>(V, W, m, a, theta, shift, D, t, xo, q, acc);
-    for (int64_t i = nq * VW + tid; i < n; i += stride)
-        residual_step<T, M, 1, WB>(V, W, m, a, theta, shift, D, t, xo, i, acc);
+    constexpr uint32_t kDs = sizeof(double) / sizeof(T);  // D bytes per T byte
+    // 16-byte groups up to 8 vectors, 8-byte groups up to 16, one element
+    // beyond (registers)
+    constexpr int VWR = M <= 8 ? 16 / (int)sizeof(T) : (M <= 16 ? 8 / (int)sizeof(T) : 1);
+    const char *Db = (const char *)D;
+    char *tb = (char *)t, *xb = (char *)xo;
+    stream<T, VWR>(n, [&](uint32_t off, auto vwt) {
+        constexpr int VW = decltype(vwt)::value;
+        typedef typename Vec<T, VW>::type VT;
+        VT vv[M];
+        double x[VW], w[VW];
+#pragma unroll
+        for (int e = 0; e < VW; ++e) x[e] = w[e] = 0;
+#pragma unroll
+        for (int j0 = 0; j0 < M; j0 += C) {
+            VT wv[C];
+#pragma unroll
+            for (int j = 0; j < C; ++j) {
+                vv[j0 + j] = ldb<VT>(V.p[j0 + j], off);
+                wv[j] = ldb<VT>(W.p[j0 + j], off);
+            }
+#pragma unroll
+            for (int j = 0; j < C; ++j)
+#pragma unroll
+                for (int e = 0; e < VW; ++e) {
+                    x[e] += a.g[j0 + j] * el<T, VW>(vv[j0 + j], e);
+                    w[e] += a.g[j0 + j] * el<T, VW>(wv[j], e);
+                }
+        }
+        typedef typename Vec<double, VW>::type DT;
+        const DT dv = ldb<DT>(Db, off * kDs);
+        VT tv, xv;
+        double td[VW];
+#pragma unroll
+        for (int e = 0; e < VW; ++e) {
+            const double r = w[e] - (theta - shift) * x[e];
+            const double d = fabs(el<double, VW>(dv, e) - theta);
+            const double tval = d > 1e-12 ? r / d : r / 1e-12;
+            set_el<T, VW>(tv, e, (T)tval);
+            set_el<T, VW>(xv, e, (T)x[e]);
+            td[e] = el<T, VW>(tv, e);
+            acc[0] += r * r;
+            acc[1] += td[e] * td[e];
+        }
+        stb<VT>(tb, off, tv);
+        if (xo) stb<VT>(xb, off, xv);
+#pragma unroll
+        for (int j = 0; j < M; ++j)
+#pragma unroll
+            for (int e = 0; e < VW; ++e) acc[2 + j] += el<T, VW>(vv[j], e) * td[e];
+        if constexpr (WB) {
+            const uint32_t off2 = launder(off);
+#pragma unroll
+            for (int j0 = 0; j0 < M; j0 += C) {
+                VT wv[C];
+#pragma unroll
+                for (int j = 0; j < C; ++j) wv[j] = ldb<VT>(W.p[j0 + j], off2);
+#pragma unroll
+                for (int j = 0; j < C; ++j)
+#pragma unroll
+                    for (int e = 0; e < VW; ++e) acc[2 + M + j0 + j] += el<T, VW>(wv[j], e) * td[e];
+            }
+        }
+    }, [&](size_t b) {
+        advance<M>(V, b);
+        advance<M>(W, b);
+        Db += b * kDs;
+        tb += b;
+        if (xo) xb += b;
+    });
     if constexpr (WB) block_partials_vw<M>(acc, m, red);
     else block_partials<M + 2>(acc, m + 2, red);
 }
 
-template <typename T, typename To, int M, int VW>
-__device__ __forceinline__ void lincomb_step(const Ptrs &X, int m, const Coef &g, To *__restrict__ out,
-                                             int64_t q)
-{
-    double s[VW];
-#pragma unroll
-    for (int e = 0; e < VW; ++e) s[e] = 0;
-#pragma unroll
-    for (int j = 0; j < M; ++j)
-        if (j < m) {
-            const auto xv = ldv<T, VW>(X.p[j], q);
-#pragma unroll
-            for (int e = 0; e < VW; ++e) s[e] += g.g[j] * el<T, VW>(xv, e);
-        }
-#pragma unroll
-    for (int e = 0; e < VW; ++e) out[q * VW + e] = (To)s[e];
-}
-
+// ---- lincomb: out = sum g_j X_j ---------------------------------------------
 template <typename T, typename To, int M>
 __global__ __launch_bounds__(kBlock) void k_lincomb(Ptrs X, int m, Coef g, To *__restrict__ out,
                                                     int64_t n)
 {
-    constexpr int VW = 16 / (int)sizeof(T);
-    const int64_t stride = (int64_t)gridDim.x * kBlock, nq = n / VW;
-    const int64_t tid = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-    for (int64_t q = tid; q < nq; q += stride) lincomb_step<T, To, M, VW>(X, m, g, out, q);
-    for (int64_t i = nq * VW + tid; i < n; i += stride) lincomb_step<T, To, M, 1>(X, m, g, out, i);
+    constexpr int C = kChunk<M>;
+    constexpr uint32_t kNum = sizeof(To), kDen = sizeof(T);  // output bytes per input bytes
+    char *ob = (char *)out;
+    stream<T, 16 / (int)sizeof(T)>(n, [&](uint32_t off, auto vwt) {
+        constexpr int VW = decltype(vwt)::value;
+        typedef typename Vec<T, VW>::type VT;
+        typedef typename Vec<To, VW>::type OT;
+        double s[VW];
+#pragma unroll
+        for (int e = 0; e < VW; ++e) s[e] = 0;
+#pragma unroll
+        for (int j0 = 0; j0 < M; j0 += C) {
+            VT xv[C];
+#pragma unroll
+            for (int j = 0; j < C; ++j) xv[j] = ldb<VT>(X.p[j0 + j], off);
+#pragma unroll
+            for (int j = 0; j < C; ++j)
+#pragma unroll
+                for (int e = 0; e < VW; ++e) s[e] += g.g[j0 + j] * el<T, VW>(xv[j], e);
+        }
+        OT o;
+#pragma unroll
+        for (int e = 0; e < VW; ++e) set_el<To, VW>(o, e, (To)s[e]);
+        stb<OT>(ob, off / kDen * kNum, o);
+    }, [&](size_t b) { advance<M>(X, b); ob += b / kDen * kNum; });
 }
 
 template <typename Ti, typename To>
@@ -507,18 +576,40 @@ __global__ __launch_bounds__(kBlock) void k_argmin_fin(const double *__restrict_
 
 } // namespace
 
+// compile-time vector counts: m rounded up to the next of 1, 2, 4, 6, 8, 12,
+// 16, 24, 32 (at most 32 with W dots) and 64
+#define DK_BUCKET32(m, CALL)                                                     \
+    do {                                                                         \
+        if ((m) <= 1) { constexpr int MM = 1; CALL; }                            \
+        else if ((m) <= 2) { constexpr int MM = 2; CALL; }                       \
+        else if ((m) <= 4) { constexpr int MM = 4; CALL; }                       \
+        else if ((m) <= 6) { constexpr int MM = 6; CALL; }                       \
+        else if ((m) <= 8) { constexpr int MM = 8; CALL; }                       \
+        else if ((m) <= 12) { constexpr int MM = 12; CALL; }                     \
+        else if ((m) <= 16) { constexpr int MM = 16; CALL; }                     \
+        else if ((m) <= 24) { constexpr int MM = 24; CALL; }                     \
+        else { constexpr int MM = 32; CALL; }                                    \
+    } while (0)
 #define DK_BUCKET(m, CALL)                                                       \
     do {                                                                         \
-        if ((m) <= 8) { constexpr int MM = 8; CALL; }                            \
+        if ((m) <= 1) { constexpr int MM = 1; CALL; }                            \
+        else if ((m) <= 2) { constexpr int MM = 2; CALL; }                       \
+        else if ((m) <= 4) { constexpr int MM = 4; CALL; }                       \
+        else if ((m) <= 6) { constexpr int MM = 6; CALL; }                       \
+        else if ((m) <= 8) { constexpr int MM = 8; CALL; }                       \
+        else if ((m) <= 12) { constexpr int MM = 12; CALL; }                     \
         else if ((m) <= 16) { constexpr int MM = 16; CALL; }                     \
+        else if ((m) <= 24) { constexpr int MM = 24; CALL; }                     \
         else if ((m) <= 32) { constexpr int MM = 32; CALL; }                     \
         else { constexpr int MM = 64; CALL; }                                    \
     } while (0)
 
-static Ptrs make_ptrs(int m, void *const *X)
+// vector slots j >= m repeat the last vector (fill when m = 0): their
+// coefficients are 0 and their sums are not stored
+static Ptrs make_ptrs(int m, void *const *X, const void *fill = nullptr)
 {
     Ptrs p;
-    for (int j = 0; j < DK_MAX_VEC; ++j) p.p[j] = j < m ? X[j] : nullptr;
+    for (int j = 0; j < DK_MAX_VEC; ++j) p.p[j] = j < m ? X[j] : (m > 0 ? X[m - 1] : fill);
     return p;
 }
 static Coef make_coef(int m, const double *g)
@@ -538,7 +629,7 @@ extern "C" int dk_multidot(int m, void *const *X, const void *y, int64_t n, int 
 {
     if (m < 0 || m > DK_MAX_VEC) return (int)hipErrorInvalidValue;
     const int nb = grid_for(n);
-    const Ptrs P = make_ptrs(m, X);
+    const Ptrs P = make_ptrs(m, X, y);
     if (prec == 0)
         DK_BUCKET(m, hipLaunchKernelGGL((k_multidot<double, MM>), dim3(nb), dim3(kBlock), 0, s, P, m,
                                         (const double *)y, n, red));
@@ -554,7 +645,7 @@ extern "C" int dk_ortho(int m, void *const *X, const double *g, void *y, int64_t
 {
     if (m < 0 || m > DK_MAX_VEC) return (int)hipErrorInvalidValue;
     const int nb = grid_for(n);
-    const Ptrs P = make_ptrs(m, X);
+    const Ptrs P = make_ptrs(m, X, y);
     const Coef G = make_coef(m, g);
     if (prec == 0)
         DK_BUCKET(m, hipLaunchKernelGGL((k_ortho<double, MM>), dim3(nb), dim3(kBlock), 0, s, P, m, G,
@@ -578,12 +669,6 @@ extern "C" int dk_residual(int m, void *const *V, void *const *W, const double *
 #define DK_RES_LAUNCH(T, WB)                                                                  \
     hipLaunchKernelGGL((k_residual<T, MM, WB>), dim3(nb), dim3(kBlock), 0, s, PV, PW, m, A, theta, \
                        shift, D, (T *)t, (T *)x_out, n, red)
-#define DK_BUCKET32(m, CALL)                                  \
-    do {                                                      \
-        if ((m) <= 8) { constexpr int MM = 8; CALL; }         \
-        else if ((m) <= 16) { constexpr int MM = 16; CALL; }  \
-        else { constexpr int MM = 32; CALL; }                 \
-    } while (0)
     if (wdots) {
         if (prec == 0) DK_BUCKET32(m, DK_RES_LAUNCH(double, true));
         else DK_BUCKET32(m, DK_RES_LAUNCH(float, true));
@@ -591,7 +676,6 @@ extern "C" int dk_residual(int m, void *const *V, void *const *W, const double *
         if (prec == 0) DK_BUCKET(m, DK_RES_LAUNCH(double, false));
         else DK_BUCKET(m, DK_RES_LAUNCH(float, false));
     }
-#undef DK_BUCKET32
 #undef DK_RES_LAUNCH
     int e = (int)hipGetLastError();
     return e ? e : finalize(red, nb, wdots ? 2 + 2 * m : m + 2, out, s);
