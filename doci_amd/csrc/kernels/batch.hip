@@ -40,6 +40,7 @@
 // Vectors are stored unnormalised with a scale per basis vector (V_j =
 // s_j * stored_j), which saves a normalisation pass.
 #include <hip/hip_runtime.h>
+#include <float.h>
 #include <math.h>
 
 #include "../dk.h"
@@ -417,13 +418,18 @@ __device__ int take_slot(uint64_t &used, int lo, int count)
 // Cyclic Jacobi on the leading n x n block of A (row-major, stride LDY),
 // rotations accumulated into Y; pairs of a sweep run in parallel in the
 // round-robin (tournament) order, so the disjoint rotations of one step
-// commute.  Stops when the off-diagonal mass is below 1e-32 of the total.
+// commute.  A pair is rotated only while |a_pq| > eps sqrt(|a_pp a_qq|)
+// (below that its effect on the eigenvalues is under the rounding of the
+// diagonal); stops after a sweep without a rotation, or when the
+// off-diagonal mass is below 1e-32 of the total.  (Without the threshold,
+// off-diagonal entries at the rounding level can keep every sweep rotating
+// until the sweep limit.)
 __device__ void jacobi_par(double *A, double *Y, int n)
 {
     __shared__ double cs[DK_BMAX_VEC / 2][2];
     __shared__ int pr[DK_BMAX_VEC / 2][2];
     __shared__ double red[2][kSmall];
-    __shared__ int done;
+    __shared__ int done, rotated;
     const int nn = n + (n & 1);
     const int npair = nn / 2;
     for (int sweep = 0; sweep < 40; ++sweep) {
@@ -444,6 +450,7 @@ __device__ void jacobi_par(double *A, double *Y, int n)
                 a += red[1][q];
             }
             done = (o == 0 || o <= 1e-32 * a);
+            rotated = 0;
         }
         __syncthreads();
         if (done) break;
@@ -466,12 +473,13 @@ __device__ void jacobi_par(double *A, double *Y, int n)
                 }
                 double c = 1, s = 0;
                 if (q < n) {
-                    const double apq = A[p * LDY + q];
-                    if (apq != 0) {
-                        const double tau = (A[q * LDY + q] - A[p * LDY + p]) / (2 * apq);
+                    const double apq = A[p * LDY + q], app = A[p * LDY + p], aqq = A[q * LDY + q];
+                    if (apq != 0 && fabs(apq) > DBL_EPSILON * sqrt(fabs(app) * fabs(aqq))) {
+                        const double tau = (aqq - app) / (2 * apq);
                         const double t = (tau >= 0 ? 1.0 : -1.0) / (fabs(tau) + sqrt(1 + tau * tau));
                         c = 1 / sqrt(1 + t * t);
                         s = t * c;
+                        if (s != 0) rotated = 1;
                     }
                 }
                 pr[k][0] = p;
@@ -507,6 +515,7 @@ __device__ void jacobi_par(double *A, double *Y, int n)
             }
             __syncthreads();
         }
+        if (!rotated) break; // read by all after the step's barrier, reset after the next one
     }
 }
 
