@@ -123,6 +123,12 @@ constexpr int vw_for()
     return M <= 16 ? w : (M <= 32 ? (w >= 2 ? w / 2 : 1) : (w >= 4 ? w / 4 : 1));
 }
 
+// Vectors are loaded in groups of kGrp, all loads of a group issued
+// together: a group past the instance's m is skipped (a uniform branch per
+// group), slots of a group past m repeat the instance's last vector with
+// coefficient 0 (see k_bstream).
+constexpr int kGrp = 4;
+
 // Rows [r0, r1) of one tile: steps of VW rows (chunk index q = row / VW),
 // then the scalar tail.
 #define DK_ROWS(STEP)                                                                              \
@@ -143,11 +149,14 @@ __device__ __forceinline__ void ortho_rows(const T *const (&V)[M], int m, const 
 #pragma unroll
     for (int e = 0; e < W; ++e) td[e] = el<T, W>(tv, e);
 #pragma unroll
-    for (int j = 0; j < M; ++j)
-        if (j < m) {
-            xv[j] = ldv<T, W>(V[j], q);
+    for (int j0 = 0; j0 < M; j0 += kGrp)
+        if (j0 < m) {
 #pragma unroll
-            for (int e = 0; e < W; ++e) td[e] -= g[j] * el<T, W>(xv[j], e);
+            for (int j = j0; j < j0 + kGrp; ++j) xv[j] = ldv<T, W>(V[j], q);
+#pragma unroll
+            for (int j = j0; j < j0 + kGrp; ++j)
+#pragma unroll
+                for (int e = 0; e < W; ++e) td[e] -= g[j] * el<T, W>(xv[j], e);
         }
 #pragma unroll
     for (int e = 0; e < W; ++e) {
@@ -157,23 +166,35 @@ __device__ __forceinline__ void ortho_rows(const T *const (&V)[M], int m, const 
     }
     if (m > 0) stv<T, W>(t, q, tv);
 #pragma unroll
-    for (int j = 0; j < M; ++j)
-        if (j < m)
+    for (int j0 = 0; j0 < M; j0 += kGrp)
+        if (j0 < m)
 #pragma unroll
-            for (int e = 0; e < W; ++e) acc[1 + j] += el<T, W>(xv[j], e) * td[e];
+            for (int j = j0; j < j0 + kGrp; ++j)
+#pragma unroll
+                for (int e = 0; e < W; ++e) acc[1 + j] += el<T, W>(xv[j], e) * td[e];
 }
 
 template <typename T, int M, int W>
 __device__ __forceinline__ void scol_rows(const T *const (&Vp)[M + 1], int m, const T *Wm, uint32_t q,
                                           double (&acc)[M + 1])
 {
-    const auto wv = ldv<T, W>(Wm, q);
+    typedef typename Vec<T, W>::type VT;
+    const VT wv = ldv<T, W>(Wm, q);
+    {
+        const VT vv = ldv<T, W>(Vp[0], q);
 #pragma unroll
-    for (int j = 0; j <= M; ++j)
-        if (j <= m) {
-            const auto vv = ldv<T, W>(Vp[j], q);
+        for (int e = 0; e < W; ++e) acc[0] += el<T, W>(vv, e) * el<T, W>(wv, e);
+    }
 #pragma unroll
-            for (int e = 0; e < W; ++e) acc[j] += el<T, W>(vv, e) * el<T, W>(wv, e);
+    for (int j0 = 1; j0 <= M; j0 += kGrp)
+        if (j0 <= m) {
+            VT vv[kGrp];
+#pragma unroll
+            for (int j = 0; j < kGrp; ++j) vv[j] = ldv<T, W>(Vp[j0 + j], q);
+#pragma unroll
+            for (int j = 0; j < kGrp; ++j)
+#pragma unroll
+                for (int e = 0; e < W; ++e) acc[j0 + j] += el<T, W>(vv[j], e) * el<T, W>(wv, e);
         }
 }
 
@@ -186,14 +207,21 @@ __device__ __forceinline__ void deflate_rows(const T *const (&V)[M], const T *co
 #pragma unroll
     for (int e = 0; e < W; ++e) a[e] = b[e] = 0;
 #pragma unroll
-    for (int j = 0; j < M; ++j)
-        if (j < m) {
-            const VT vv = ldv<T, W>(V[j], q), wv = ldv<T, W>(Wv[j], q);
+    for (int j0 = 0; j0 < M; j0 += kGrp)
+        if (j0 < m) {
+            VT vv[kGrp], wv[kGrp];
 #pragma unroll
-            for (int e = 0; e < W; ++e) {
-                a[e] += c[j] * el<T, W>(vv, e);
-                b[e] += c[j] * el<T, W>(wv, e);
+            for (int j = 0; j < kGrp; ++j) {
+                vv[j] = ldv<T, W>(V[j0 + j], q);
+                wv[j] = ldv<T, W>(Wv[j0 + j], q);
             }
+#pragma unroll
+            for (int j = 0; j < kGrp; ++j)
+#pragma unroll
+                for (int e = 0; e < W; ++e) {
+                    a[e] += c[j0 + j] * el<T, W>(vv[j], e);
+                    b[e] += c[j0 + j] * el<T, W>(wv[j], e);
+                }
         }
     VT ov, ow;
 #pragma unroll
@@ -217,15 +245,21 @@ __device__ __forceinline__ void resid_rows(const T *const (&V)[M], const T *cons
 #pragma unroll
     for (int e = 0; e < W; ++e) xs[e] = ws[e] = 0;
 #pragma unroll
-    for (int j = 0; j < M; ++j)
-        if (j < m) {
-            vv[j] = ldv<T, W>(V[j], q);
-            const VT wv = ldv<T, W>(Wv[j], q);
+    for (int j0 = 0; j0 < M; j0 += kGrp)
+        if (j0 < m) {
+            VT wv[kGrp];
 #pragma unroll
-            for (int e = 0; e < W; ++e) {
-                xs[e] += yc[j] * el<T, W>(vv[j], e);
-                ws[e] += yc[j] * el<T, W>(wv, e);
+            for (int j = j0; j < j0 + kGrp; ++j) {
+                vv[j] = ldv<T, W>(V[j], q);
+                wv[j - j0] = ldv<T, W>(Wv[j], q);
             }
+#pragma unroll
+            for (int j = j0; j < j0 + kGrp; ++j)
+#pragma unroll
+                for (int e = 0; e < W; ++e) {
+                    xs[e] += yc[j] * el<T, W>(vv[j], e);
+                    ws[e] += yc[j] * el<T, W>(wv[j - j0], e);
+                }
         }
     VT tv;
     double td[W];
@@ -243,10 +277,12 @@ __device__ __forceinline__ void resid_rows(const T *const (&V)[M], const T *cons
     }
     stv<T, W>(t, q, tv);
 #pragma unroll
-    for (int j = 0; j < M; ++j)
-        if (j < m)
+    for (int j0 = 0; j0 < M; j0 += kGrp)
+        if (j0 < m)
 #pragma unroll
-            for (int e = 0; e < W; ++e) acc[2 + j] += el<T, W>(vv[j], e) * td[e];
+            for (int j = j0; j < j0 + kGrp; ++j)
+#pragma unroll
+                for (int e = 0; e < W; ++e) acc[2 + j] += el<T, W>(vv[j], e) * td[e];
 }
 
 template <typename T, int M, int OP>
@@ -268,9 +304,13 @@ __global__ __launch_bounds__(kBlock) void k_bstream(const dk_binst *__restrict__
     const uint32_t r1 = min(in.n, r0 + in.tile_rows);
     double *out = part + (size_t)(in.tile0 + ti) * DK_BPV; // the instance's rows, in tile order
 
+    // slots past m repeat the last basis vector (t itself while the basis is
+    // empty) with coefficient 0, so that every load is unconditional; their
+    // sums are not stored
     const T *V[M];
 #pragma unroll
-    for (int j = 0; j < M; ++j) V[j] = j < m ? slot_ptr<T>(in, st.vslot[j]) : nullptr;
+    for (int j = 0; j < M; ++j)
+        V[j] = m > 0 ? slot_ptr<T>(in, st.vslot[j < m ? j : m - 1]) : slot_ptr<T>(in, st.tslot);
 
     if constexpr (OP == DK_BOP_ORTHO || OP == DK_BOP_ORTHO2) {
         constexpr int VW = vw_for<T, M>();
@@ -290,7 +330,7 @@ __global__ __launch_bounds__(kBlock) void k_bstream(const dk_binst *__restrict__
         // column m: V_0..V_m (V_m is t itself) against W_m, the sigma just computed
         const T *Vp[M + 1];
 #pragma unroll
-        for (int j = 0; j <= M; ++j) Vp[j] = j <= m ? slot_ptr<T>(in, st.vslot[j]) : nullptr;
+        for (int j = 0; j <= M; ++j) Vp[j] = slot_ptr<T>(in, st.vslot[j <= m ? j : m]);
         const T *Wm = slot_ptr<T>(in, st.wslot[m]);
         double acc[M + 1];
 #pragma unroll
@@ -304,7 +344,7 @@ __global__ __launch_bounds__(kBlock) void k_bstream(const dk_binst *__restrict__
         const int keep = st.keep;
         const T *W[M];
 #pragma unroll
-        for (int j = 0; j < M; ++j) W[j] = j < m ? slot_ptr<T>(in, st.wslot[j]) : nullptr;
+        for (int j = 0; j < M; ++j) W[j] = slot_ptr<T>(in, st.wslot[j < m ? j : m - 1]);
         for (int k = 0; k < keep; ++k) {
             double c[M];
 #pragma unroll
@@ -324,7 +364,7 @@ __global__ __launch_bounds__(kBlock) void k_bstream(const dk_binst *__restrict__
 #pragma unroll
         for (int j = 0; j < M; ++j) {
             yc[j] = j < m ? st.y[j] : 0.0;
-            W[j] = j < m ? slot_ptr<T>(in, st.wslot[j]) : nullptr;
+            W[j] = slot_ptr<T>(in, st.wslot[j < m ? j : m - 1]);
         }
         double acc[M + 2];
 #pragma unroll
