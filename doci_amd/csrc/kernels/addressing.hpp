@@ -132,8 +132,12 @@ __device__ __forceinline__ void for_each_hop_pred(uint32_t r, uint64_t m, int L,
 // with the prefix sums over the occupied orbitals
 //   Qd_e = sum_{e'<e} C(L-1-o_e', N-e') - C(L-1-o_e', N-e'-1),
 //   Qu_e = sum_{e'<e} C(L-1-o_e', N-e'+1) - C(L-1-o_e', N-e').
-// f(o, t, J) is called for the N(L-N) hops, o ascending, then t ascending.
-template <int NM, int KM, typename F>
+// f(o, t, J, hop) is called for the N(L-N) hops (hop = true), o ascending,
+// then t ascending, in groups of G empty orbitals with no branch inside a
+// group, so that a caller's G gathers are in flight together; the calls of a
+// group past the last empty orbital have hop = false, t = 0 and a J that
+// may lie outside the vector (read it with vec_load and discard it).
+template <int NM, int KM, int G, typename F>
 __device__ __forceinline__ void for_each_hop_sep(uint32_t r, uint64_t m, int L, int N,
                                                  const Binom &C, F &&f)
 {
@@ -178,15 +182,40 @@ __device__ __forceinline__ void for_each_hop_sep(uint32_t r, uint64_t m, int L, 
                 }
             Td[k] = C(x, N - j) + sd;
             Tu[k] = C(x, N - j + 1) + su;
+        } else {
+            t[k] = 0;
+            Td[k] = Tu[k] = 0;
         }
     }
 #pragma unroll
     for (int h = 0; h < NM; ++h)
         if (h < N) {
 #pragma unroll
-            for (int k = 0; k < KM; ++k)
-                if (k < nk) f(o[h], t[k], t[k] < o[h] ? Pd[h] - Td[k] : Pu[h] - Tu[k]);
+            for (int k0 = 0; k0 < KM; k0 += G)
+                if (k0 < nk) {
+#pragma unroll
+                    for (int k = k0; k < k0 + G; ++k)
+                        f(o[h], t[k], t[k] < o[h] ? Pd[h] - Td[k] : Pu[h] - Tu[k], k < nk);
+                }
         }
+}
+
+// A vector addressed through a buffer resource: a load at any 32-bit byte
+// offset is safe (past the vector it returns 0), so a walk may issue loads
+// for steps it then discards without computing an in-range index for them.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t vec_rsrc(const void *base, uint64_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc((void *)base, (short)0,
+                                             (int)(bytes < 0xffffffffull ? bytes : 0xffffffffull),
+                                             0x00020000);
+}
+template <typename T>
+__device__ __forceinline__ double vec_load(__amdgpu_buffer_rsrc_t r, uint32_t i)
+{
+    if constexpr (sizeof(T) == 8)
+        return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)(i * 8u), 0, 0));
+    else
+        return (double)__builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)(i * 4u), 0, 0));
 }
 
 // Diagonal entry of determinant mask m (src/doci.c:589-624), the reference's

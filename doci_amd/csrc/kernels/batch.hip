@@ -378,6 +378,9 @@ __global__ __launch_bounds__(kBlock) void k_bstream(const dk_binst *__restrict__
 #undef DK_ROWS
 
 // ---- batched lane sigma ----------------------------------------------------
+// hops whose gathers are issued together in the separable walk
+constexpr int kHopGroup = 4;
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_bsigma_lane(const dk_binst *__restrict__ insts,
                                                         const dk_bstate *__restrict__ states,
@@ -400,12 +403,17 @@ __global__ __launch_bounds__(kBlock) void k_bsigma_lane(const dk_binst *__restri
     const dk::Binom C{bt, N2};
     const uint32_t r0 = ti * in.tile_rows;
     const uint32_t r1 = min(in.n, r0 + in.tile_rows);
+    // gathers through a buffer resource: 32-bit offsets, and the discarded
+    // steps of a group need no in-range index
+    const __amdgpu_buffer_rsrc_t rc = dk::vec_rsrc(c, (uint64_t)in.n * sizeof(T));
     for (uint32_t I = r0 + threadIdx.x; I < r1; I += kBlock) {
         const uint64_t msk = dk::unrank(I, L, N, C);
         double acc = (in.D[I] - shift) * (double)c[I];
         if (N <= 12 && L - N <= 16) { // uniform: one instance per workgroup
-            dk::for_each_hop_sep<12, 16>(I, msk, L, N, C, [&](int o, int tt, uint32_t Jr) {
-                acc += K[o * L + tt] * (double)c[Jr];
+            dk::for_each_hop_sep<12, 16, kHopGroup>(I, msk, L, N, C,
+                                                    [&](int o, int tt, uint32_t Jr, bool hop) {
+                const double v = K[o * L + tt] * dk::vec_load<T>(rc, Jr);
+                acc += hop ? v : 0.0;
             });
         } else {
             // branch-free walk: a step whose t is occupied reads c[I] and adds 0

@@ -62,8 +62,12 @@ __global__ __launch_bounds__(kBlock) void k_sigma_lane(dk_tables t, const double
     const uint64_t m = dk::unrank(I, L, N, C);
     double acc = (D[I] - shift) * (double)c[I];
     if (kSep) {
-        dk::for_each_hop_sep<12, 16>(I, m, L, N, C, [&](int o, int tt, uint32_t Jr) {
-            acc += K[o * L + tt] * (double)c[Jr];
+        // four gathers in flight per group of the walk, through a buffer
+        // resource (the discarded steps of a group need no in-range index)
+        const __amdgpu_buffer_rsrc_t rc = dk::vec_rsrc(c, (uint64_t)t.n * sizeof(T));
+        dk::for_each_hop_sep<12, 16, 4>(I, m, L, N, C, [&](int o, int tt, uint32_t Jr, bool hop) {
+            const double v = K[o * L + tt] * dk::vec_load<T>(rc, Jr);
+            acc += hop ? v : 0.0;
         });
     } else {
         // branch-free walk: a step whose t is occupied reads c[I] and adds 0
