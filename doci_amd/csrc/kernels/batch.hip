@@ -435,11 +435,21 @@ __device__ void reduce_tiles(const dk_binst &in, const double *__restrict__ part
     double acc[DK_BPV];
 #pragma unroll
     for (int k = 0; k < DK_BPV; ++k) acc[k] = 0;
+    // whole rows, loaded without per-value branches so that they are in
+    // flight together (values k >= nv of a row are never written by the
+    // streaming passes; their sums are not read)
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    static_assert(DK_BPV % 2 == 0, "rows of 16-byte pairs");
     for (uint32_t t = threadIdx.x; t < in.ntiles; t += kSmall) {
-        const double *p = part + (size_t)(in.tile0 + t) * DK_BPV;
+        const d2 *p = (const d2 *)(part + (size_t)(in.tile0 + t) * DK_BPV);
+        d2 v[DK_BPV / 2];
 #pragma unroll
-        for (int k = 0; k < DK_BPV; ++k)
-            if (k < nv) acc[k] += p[k];
+        for (int k = 0; k < DK_BPV / 2; ++k) v[k] = p[k];
+#pragma unroll
+        for (int k = 0; k < DK_BPV / 2; ++k) {
+            acc[2 * k] += v[k].x;
+            acc[2 * k + 1] += v[k].y;
+        }
     }
 #pragma unroll
     for (int k = 0; k < DK_BPV; ++k) sh[k][threadIdx.x] = acc[k];
