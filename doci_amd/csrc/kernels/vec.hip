@@ -39,12 +39,6 @@ inline int grid_for(int64_t n)
     return (int)b;
 }
 
-template <typename T>
-__device__ __forceinline__ double ld(const void *p, int64_t i)
-{
-    return (double)((const T *)p)[i];
-}
-
 // Reduce acc[0..nv) over the block and store one partial per value at
 // red[blockIdx.x * nv + k].
 template <int NV>
