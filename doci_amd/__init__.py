@@ -73,7 +73,7 @@ EXPORTS = [
     "doci_read_fcidump", "doci_integral_tables", "doci_free", "doci_dev_copy", "doci_op_set_kernel",
     "doci_op_slab_split", "doci_op_check_violations", "doci_op_tune", "doci_op_time_sigma",
     "doci_op_star_split", "doci_op_kernel", "doci_batch_defaults", "doci_batch_create", "doci_batch_free",
-    "doci_batch_footprint", "doci_batch_add", "doci_batch_step", "doci_batch_run",
+    "doci_batch_footprint", "doci_batch_add", "doci_batch_add_many", "doci_batch_step", "doci_batch_run",
     "doci_batch_result_of", "doci_batch_release", "doci_batch_clock", "doci_batch_stats",
 ]
 
@@ -137,6 +137,9 @@ def lib():
     l.doci_batch_footprint.restype = C.c_size_t
     l.doci_batch_add.argtypes = [vp, C.c_uint, C.c_uint, C.c_double, _dp, _dp, C.c_int,
                                  C.POINTER(C.c_int)]
+    l.doci_batch_add_many.argtypes = [vp, C.c_int, C.POINTER(C.c_uint), C.POINTER(C.c_uint),
+                                      C.POINTER(C.c_double), C.POINTER(vp), C.POINTER(vp), C.c_int,
+                                      C.POINTER(C.c_int)]
     l.doci_batch_step.argtypes = [vp, C.POINTER(C.c_int), C.POINTER(C.c_int)]
     l.doci_batch_run.argtypes = [vp]
     l.doci_batch_result_of.argtypes = [vp, C.c_int, C.POINTER(BatchResult), vp]
@@ -417,15 +420,37 @@ class Batch:
         core, one, two = gen_integrals(L, seed)
         return self.add(L, N, core, one, two, precision)
 
+    def add_many(self, items, precision: str = "fp64") -> list:
+        """Queues (L, N, core, one, two) tuples in one call (the library
+        transforms their integrals on several threads).  Returns the ids."""
+        items = list(items)
+        k = len(items)
+        if k == 0:
+            return []
+        arrs = [(np.ascontiguousarray(it[3], np.float64), np.ascontiguousarray(it[4], np.float64))
+                for it in items]
+        Ls = (C.c_uint * k)(*[int(it[0]) for it in items])
+        Ns = (C.c_uint * k)(*[int(it[1]) for it in items])
+        cores = (C.c_double * k)(*[float(it[2]) for it in items])
+        ones = (C.c_void_p * k)(*[a[0].ctypes.data for a in arrs])
+        twos = (C.c_void_p * k)(*[a[1].ctypes.data for a in arrs])
+        ids = (C.c_int * k)()
+        _check(lib().doci_batch_add_many(self._h, k, Ls, Ns, cores, ones, twos,
+                                         1 if precision == "mixed" else 0, ids), "doci_batch_add_many")
+        for it, i in zip(items, ids):
+            self.dims[i] = binom(int(it[1]), int(it[0]))
+        return list(ids)
+
     def add_pipelined(self, items, precision: str = "fp64", chunk: int = 64) -> list:
         """Queues many instances ((L, N, core, one, two) tuples) in chunks,
         stepping the engine after each chunk but the last, so that the host
-        intake of a chunk (the integral transform) overlaps the GPU's
-        iterations of the chunks before it.  Returns the instance ids."""
+        intake of a chunk (the integral transform, on several threads)
+        overlaps the GPU's iterations of the chunks before it.  Returns the
+        instance ids."""
         items = list(items)
         ids = []
         for c0 in range(0, len(items), chunk):
-            ids += [self.add(*it, precision=precision) for it in items[c0:c0 + chunk]]
+            ids += self.add_many(items[c0:c0 + chunk], precision=precision)
             if c0 + chunk < len(items):
                 self.step()
         return ids

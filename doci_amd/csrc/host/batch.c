@@ -406,8 +406,7 @@ void doci_batch_free(doci_batch *b)
     free(b);
 }
 
-int doci_batch_add(doci_batch *b, unsigned L, unsigned N, double core, const double *one,
-                   const double *two, int precision, int *id)
+static int check_instance(const doci_batch *b, unsigned L, unsigned N)
 {
     if (L == 0 || L > DK_MAX_L) return doci_fail(DOCI_E_ORBITALS, "Too many orbitals (L=%u).", L);
     if (N == 0 || N > L) return doci_fail(DOCI_E_DIM, "invalid number of pairs N=%u", N);
@@ -416,32 +415,124 @@ int doci_batch_add(doci_batch *b, unsigned L, unsigned N, double core, const dou
         return doci_fail(DOCI_E_DIM, "C(%u,%u) = %llu exceeds 2^31-1", L, N, (unsigned long long)n);
     if (doci_batch_footprint(&b->o, L, N) > b->o.mem_budget)
         return doci_fail(DOCI_E_NOMEM, "instance L=%u N=%u needs more than the memory budget", L, N);
-    if (b->nrec == b->rec_cap) {
-        const int nc = b->rec_cap ? 2 * b->rec_cap : 64;
-        brec *nr = realloc(b->rec, sizeof(brec) * nc);
-        if (!nr) return doci_fail(DOCI_E_NOMEM, "batch records");
-        b->rec = nr;
-        b->rec_cap = nc;
-    }
-    brec *r = &b->rec[b->nrec];
-    memset(r, 0, sizeof *r);
-    /* transform now (request intake): only the O(L^2) tables are kept */
+    return DOCI_OK;
+}
+
+static int reserve_records(doci_batch *b, int count)
+{
+    if (b->nrec + count <= b->rec_cap) return DOCI_OK;
+    int nc = b->rec_cap ? 2 * b->rec_cap : 64;
+    while (nc < b->nrec + count) nc *= 2;
+    brec *nr = realloc(b->rec, sizeof(brec) * nc);
+    if (!nr) return doci_fail(DOCI_E_NOMEM, "batch records");
+    b->rec = nr;
+    b->rec_cap = nc;
+    return DOCI_OK;
+}
+
+/* The request intake of one record: only the O(L^2) tables of the raw
+ * integrals are kept (thread-safe: touches nothing but r). */
+static int rec_tables(brec *r, unsigned L, unsigned N, const double *one, const double *two)
+{
     const size_t l = L, l2 = l * l;
     r->tab = malloc(table_bytes(L, N));
-    if (!r->tab) return doci_fail(DOCI_E_NOMEM, "instance tables");
+    if (!r->tab) return DOCI_E_NOMEM;
     double *K = (double *)r->tab, *J = K + l2, *X = J + l2, *h = X + l2;
     if (dh_build_tables(L, N, one, two, K, J, X, h, (uint32_t *)(h + l)) != DOCI_OK) {
         free(r->tab);
-        return doci_fail(DOCI_E_NOMEM, "instance tables");
+        r->tab = NULL;
+        return DOCI_E_NOMEM;
     }
+    return DOCI_OK;
+}
+
+static void rec_fill(doci_batch *b, brec *r, unsigned L, unsigned N, double core, int precision)
+{
     r->L = L;
     r->N = N;
-    r->n = (uint32_t)n;
+    r->n = (uint32_t)dh_binom(N, L);
     r->core = core;
     r->want_mixed = precision != 0;
     r->slot = -1;
     r->res.t_submit = doci_batch_clock(b);
+}
+
+int doci_batch_add(doci_batch *b, unsigned L, unsigned N, double core, const double *one,
+                   const double *two, int precision, int *id)
+{
+    int rc = check_instance(b, L, N);
+    if (rc == DOCI_OK) rc = reserve_records(b, 1);
+    if (rc != DOCI_OK) return rc;
+    brec *r = &b->rec[b->nrec];
+    memset(r, 0, sizeof *r);
+    if (rec_tables(r, L, N, one, two) != DOCI_OK) return doci_fail(DOCI_E_NOMEM, "instance tables");
+    rec_fill(b, r, L, N, core, precision);
     *id = b->nrec++;
+    return DOCI_OK;
+}
+
+/* doci_batch_add_many: the intake of the instances split over threads, each
+ * building the tables of a contiguous share of them. */
+typedef struct {
+    brec *rec;
+    const unsigned *L, *N;
+    const double *const *one, *const *two;
+    int k0, k1, rc;
+} intake_job;
+
+static void *intake_worker(void *arg)
+{
+    intake_job *j = arg;
+    for (int k = j->k0; k < j->k1 && j->rc == DOCI_OK; ++k)
+        j->rc = rec_tables(&j->rec[k], j->L[k], j->N[k], j->one[k], j->two[k]);
+    return NULL;
+}
+
+#define INTAKE_THREADS 16
+
+int doci_batch_add_many(doci_batch *b, int count, const unsigned *L, const unsigned *N,
+                        const double *core, const double *const *one, const double *const *two,
+                        int precision, int *ids)
+{
+    if (count < 0 || (count > 0 && (!L || !N || !core || !one || !two || !ids)))
+        return doci_fail(DOCI_E_ARG, "doci_batch_add_many: bad arguments");
+    for (int k = 0; k < count; ++k) {
+        const int rc = check_instance(b, L[k], N[k]);
+        if (rc != DOCI_OK) return rc;
+    }
+    int rc = reserve_records(b, count);
+    if (rc != DOCI_OK) return rc;
+    brec *r = &b->rec[b->nrec];
+    memset(r, 0, sizeof(brec) * (size_t)count);
+    /* one thread per 16 instances, at most INTAKE_THREADS (the caller's
+     * thread takes the first share) */
+    int nt = (count + 15) / 16;
+    if (nt > INTAKE_THREADS) nt = INTAKE_THREADS;
+    if (nt < 1) nt = 1;
+    intake_job job[INTAKE_THREADS];
+    pthread_t th[INTAKE_THREADS];
+    int started[INTAKE_THREADS] = {0};
+    for (int t = 0; t < nt; ++t) {
+        job[t] = (intake_job){r, L, N, one, two, (int)((int64_t)count * t / nt),
+                              (int)((int64_t)count * (t + 1) / nt), DOCI_OK};
+        if (t > 0) started[t] = pthread_create(&th[t], NULL, intake_worker, &job[t]) == 0;
+    }
+    intake_worker(&job[0]);
+    for (int t = 1; t < nt; ++t) {
+        if (started[t]) pthread_join(th[t], NULL);
+        else intake_worker(&job[t]); /* no thread: this one does the share */
+    }
+    rc = DOCI_OK;
+    for (int t = 0; t < nt; ++t)
+        if (job[t].rc != DOCI_OK) rc = job[t].rc;
+    if (rc != DOCI_OK) {
+        for (int k = 0; k < count; ++k) free(r[k].tab);
+        return doci_fail(DOCI_E_NOMEM, "instance tables");
+    }
+    for (int k = 0; k < count; ++k) {
+        rec_fill(b, &r[k], L[k], N[k], core[k], precision);
+        ids[k] = b->nrec++;
+    }
     return DOCI_OK;
 }
 

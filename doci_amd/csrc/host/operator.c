@@ -75,21 +75,31 @@ static double eri_unpacked(const double *two, size_t L, size_t i, size_t j, size
     const size_t L2 = L * L, L3 = L2 * L;
     const size_t pm[8][4] = {{i, j, k, l}, {k, l, i, j}, {j, i, l, k}, {l, k, j, i},
                              {j, i, k, l}, {l, k, i, j}, {i, j, l, k}, {k, l, j, i}};
-    int best = -1;
+    /* the loop-visited permutations in visiting (lexicographic) order, read
+     * until the first non-zero one: for a symmetric tensor that is a single
+     * read instead of one per permutation (the reads are cache misses) */
+    int ord[8], nv = 0;
     for (int q = 0; q < 8; ++q) {
-        const size_t a = pm[q][0], b = pm[q][1], c = pm[q][2], d = pm[q][3];
-        if (!(a >= b && a >= c && c >= d)) continue;
-        if (two[a + L * b + L2 * c + L3 * d] == 0) continue;
-        int earlier = best < 0;
-        for (int e = 0; e < 4 && !earlier; ++e) {
-            if (pm[q][e] == pm[best][e]) continue;
-            earlier = pm[q][e] < pm[best][e];
-            break;
+        const size_t *t = pm[q];
+        if (!(t[0] >= t[1] && t[0] >= t[2] && t[2] >= t[3])) continue;
+        int pos = nv; /* insertion by lexicographic order (equal tuples: either order) */
+        while (pos > 0) {
+            const size_t *u = pm[ord[pos - 1]];
+            int e = 0;
+            while (e < 3 && t[e] == u[e]) ++e;
+            if (!(t[e] < u[e])) break;
+            ord[pos] = ord[pos - 1];
+            --pos;
         }
-        if (earlier) best = q;
+        ord[pos] = q;
+        ++nv;
     }
-    const size_t *t = best < 0 ? pm[0] : pm[best];
-    return two[t[0] + L * t[1] + L2 * t[2] + L3 * t[3]];
+    for (int r = 0; r < nv; ++r) {
+        const size_t *t = pm[ord[r]];
+        const double v = two[t[0] + L * t[1] + L2 * t[2] + L3 * t[3]];
+        if (v != 0) return v;
+    }
+    return two[i + L * j + L2 * k + L3 * l]; /* no visited tuple is non-zero: the raw entry */
 }
 
 /* The DOCI tables of raw FCIDUMP integrals (one[L^2], two[L^4] in the

@@ -312,6 +312,12 @@ size_t doci_batch_footprint(const doci_batch_opts *o, unsigned L, unsigned N);
  * precision 0 = FP64, 1 = mixed.  *id identifies it in later calls. */
 int doci_batch_add(doci_batch *b, unsigned L, unsigned N, double core, const double *one,
                    const double *two, int precision, int *id);
+/* The same for count instances at once (instance k: L[k], N[k], core[k],
+ * one[k], two[k]); the host transform of their integrals runs on several
+ * threads.  ids[k] receives instance k's id; nothing is queued on error. */
+int doci_batch_add_many(doci_batch *b, int count, const unsigned *L, const unsigned *N,
+                        const double *core, const double *const *one, const double *const *two,
+                        int precision, int *ids);
 /* Admit what fits, run one Davidson iteration of every running instance,
  * retire finished ones.  *running / *waiting (may be NULL) receive the
  * counts afterwards. */

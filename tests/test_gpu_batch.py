@@ -261,3 +261,28 @@ def test_batch_add_pipelined_matches_all_at_once(singles):
             assert r.status == 2 and r.residual <= 1e-8, c
             assert abs(r.energy - singles[c].energy) <= 1e-9, c
             assert abs(abs(x @ singles[c].x) - 1) <= 1e-8, c
+
+
+def test_batch_add_many_equals_add():
+    """doci_batch_add_many (the integrals of 40 instances transformed on
+    three threads) queues the same instances as 40 doci_batch_add calls:
+    same iterations and energies; a bad instance in the list queues none."""
+    print("[case] batch add_many", flush=True)
+    rng = np.random.default_rng(5)
+    shapes = [(int(L), int(rng.integers(2, L // 2 + 1))) for L in rng.integers(8, 15, 40)]
+    items = [(L, N) + tuple(da.gen_integrals(L, 100 + k)) for k, (L, N) in enumerate(shapes)]
+    res = []
+    for many in (False, True):
+        with da.Batch(tile_rows=512) as b:
+            ids = b.add_many(items) if many else [b.add(*it) for it in items]
+            b.run()
+            res.append([b.result(i)[0] for i in ids])
+    for k, (r0, r1) in enumerate(zip(*res)):
+        assert r0.status == 2 and r1.status == 2, k
+        assert r0.iterations == r1.iterations, k
+        assert abs(r0.energy - r1.energy) <= 1e-12, k
+    with da.Batch() as b:
+        bad = items[:3] + [(4, 5) + tuple(da.gen_integrals(4, 1))]
+        with pytest.raises(da.DociError):
+            b.add_many(bad)
+        assert b.add_many(items[:2]) == [0, 1]
