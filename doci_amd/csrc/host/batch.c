@@ -350,8 +350,8 @@ static void group_free(bgroup *g)
     if (g->d_list) hipFree(g->d_list);
     if (g->d_tiles) hipFree(g->d_tiles);
     if (g->d_part) hipFree(g->d_part);
-    free(g->h_list);
-    free(g->h_tiles);
+    if (g->h_list) hipHostFree(g->h_list);
+    if (g->h_tiles) hipHostFree(g->h_tiles);
     memset(g, 0, sizeof *g);
 }
 
@@ -781,8 +781,15 @@ static int grow(void **h, void **d, size_t *cap, size_t need, size_t elem)
     size_t nc = *cap ? *cap : 64;
     while (nc < need) nc *= 2;
     if (h) {
-        void *nh = realloc(*h, elem * nc);
-        if (!nh) return doci_fail(DOCI_E_NOMEM, "batch lists");
+        /* pinned: the copies of these lists are enqueued asynchronously (a
+         * copy from pageable memory would hold the host for its duration) */
+        void *nh = NULL;
+        if (hipHostMalloc(&nh, elem * nc, 0) != hipSuccess || !nh)
+            return doci_fail(DOCI_E_NOMEM, "batch lists");
+        if (*h) {
+            memcpy(nh, *h, elem * *cap);
+            hipHostFree(*h);
+        }
         *h = nh;
     }
     if (*d) hipFree(*d);
