@@ -1239,6 +1239,7 @@ int doci_batch_step(doci_batch *b, int *running, int *waiting)
     const double ts0 = dh_now();
     int rc = drain(b);
     const double ts1 = dh_now();
+    double ts_rb = ts1;
     if (rc != DOCI_OK) return rc;
     int nrun = 0;
     for (int s = 0; s < b->cap; ++s) nrun += b->slot_rec[s] >= 0;
@@ -1258,6 +1259,7 @@ int doci_batch_step(doci_batch *b, int *running, int *waiting)
             rc = rebuild(b);
             if (rc != DOCI_OK) return rc;
         }
+        ts_rb = dh_now();
         for (int p = 0; p < 2; ++p) {
             bgroup *g = &b->grp[p];
             if (!g->n) continue;
@@ -1280,8 +1282,9 @@ int doci_batch_step(doci_batch *b, int *running, int *waiting)
         if (rc != DOCI_OK) return rc;
     }
     if (trace_on())
-        fprintf(stderr, "[batch] step %d: drain %.2f ms, launch %.2f ms, admit %.2f ms\n",
-                b->iterations, 1e3 * (ts1 - ts0), 1e3 * (ts2 - ts1), 1e3 * (dh_now() - ts2));
+        fprintf(stderr, "[batch] step %d: drain %.2f ms, launch %.2f ms (rebuild %.2f), admit %.2f ms\n",
+                b->iterations, 1e3 * (ts1 - ts0), 1e3 * (ts2 - ts1), 1e3 * (ts_rb - ts1),
+                1e3 * (dh_now() - ts2));
     int nr = 0, nw = 0;
     for (int i = 0; i < b->nrec; ++i) {
         nr += b->rec[i].state == 1;
