@@ -243,7 +243,7 @@ def test_davidson_max_its_code(fcidump_dir):
     assert r["rc"] == 1 and r["its"] == 3  # its++ in the loop test (davidson.c:72)
 
 
-@pytest.mark.skipif(not O.ref_available(), reason="reference build absent (GPU box)")
+@pytest.mark.skipif(not O.ref_available(), reason="reference build absent")
 @pytest.mark.parametrize("name", SMALL + ["lif_631g_1"])
 def test_oracle_equals_reference_build(name, fcidump_dir):
     """Direct comparison with the reference's own operator (oracle/_ref)."""
