@@ -176,3 +176,24 @@ def test_vector_kernels_beyond_one_window(L, prec, n):
     """Longer than one window of 2^26 lane steps (the bases advance between
     windows; the tail follows the last one)."""
     _check_all(L, n, 2, prec, 7, True)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_vector_kernels_empty_basis(L, prec):
+    """m = 0: multidot and ortho return <y,y> only (their vector slots all
+    repeat y, with coefficient 0), and ortho leaves y as it is."""
+    n = 4099
+    dt = np.float64 if prec == 0 else np.float32
+    y = np.random.default_rng(3).standard_normal(n).astype(dt)
+    y64 = y.astype(np.float64)
+    with Dev() as dv:
+        dy = dv.put(y)
+        red = dv.put(np.zeros(L.dk_red_scratch_doubles(8)))
+        out = dv.put(np.zeros(8))
+        empty = (VP * 1)()
+        print(f"[case] empty basis prec={prec}", flush=True)
+        assert L.dk_multidot(0, empty, dy, n, prec, red, out, None) == 0
+        _close(dv.get(out, 1, np.float64), [y64 @ y64], [y64 @ y64], 1e-12)
+        assert L.dk_ortho(0, empty, (C.c_double * 1)(), dy, n, prec, red, out, None) == 0
+        assert np.array_equal(dv.get(dy, n, dt), y)
+        _close(dv.get(out, 1, np.float64), [y64 @ y64], [y64 @ y64], 1e-12)
