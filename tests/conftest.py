@@ -20,7 +20,7 @@ LARGE = ["lif_631g_1", "co_631g_1"]
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (HIP device)")
     config.addinivalue_line("markers", "slow: long-running case")
-    # build what is missing (the driver builds beforehand; this only covers a bare checkout)
+    # build what is missing (a built tree skips this; it covers a bare checkout)
     if not os.path.exists(os.path.join(ROOT, "oracle", "liboracle.so")):
         subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "liboracle.so"],
                        check=True)
